@@ -1,0 +1,143 @@
+/*
+ * kwgpu — C ABI of the MI355X k-space pseudospectral solver (libkwgpu.so).
+ *
+ * This is the drop-in boundary below k-wave-python's native solver path.  In the
+ * reference, kspaceFirstOrder2D/3D (kwave/kspaceFirstOrder2D.py:141-218,
+ * kwave/kspaceFirstOrder3D.py:133-211) dispatch to run_python_backend
+ * (kwave/solvers/native.py:6-30), which constructs
+ * kwave.solvers.kspace_solver.Simulation (kspace_solver.py:122-203) and calls
+ * setup()/step()/run() (kspace_solver.py:205, 691, 749).  The Python mirror
+ * (kwave_amd/solvers/kspace_solver.py) performs the same FP64 host set-up and
+ * hands the time loop to the entry points below through ctypes.
+ *
+ * Conventions: every function returns 0 on success or a negative KW_E* code;
+ * kw_last_error() holds a thread-local message.  The library owns all device
+ * memory.  Host pointers are read (or written) only during the call.  A plan is
+ * bound to one device and one stream, is not thread-safe, and its calls are
+ * synchronous as seen by the host.  A plan holds `batch` instances of the same
+ * grid shape; each instance has its own medium, sources, sensors, dt and Nt.
+ */
+#ifndef KWGPU_H
+#define KWGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KW_ABI_VERSION 1
+
+#define KW_OK 0
+#define KW_EINVAL (-1)      /* bad argument  -> ValueError   */
+#define KW_ENOMEM (-2)      /* out of memory -> MemoryError  */
+#define KW_EHIP (-3)        /* HIP failure   -> RuntimeError */
+#define KW_EUNSUPPORTED (-4) /* unsupported configuration -> ValueError */
+
+/* physics flags (kw_plan_desc.flags) */
+#define KW_NONLINEAR 1          /* B/A term, kspace_solver.py:526-550 */
+#define KW_STOKES 2             /* y = 2 absorption, kspace_solver.py:491-494 */
+#define KW_POWERLAW_ABS 4       /* tau |k|^{y-2} term, kspace_solver.py:495-498 */
+#define KW_POWERLAW_DISP 8      /* eta |k|^{y-1} term, kspace_solver.py:521-524 */
+
+/* field identifiers for kw_plan_set_field / kw_plan_read */
+#define KW_F_C0SQ 0     /* c0^2 (kspace_solver.py:662) */
+#define KW_F_RHO0 1     /* rho0 */
+#define KW_F_DTR0_X 2   /* dt / rho0_sg_x (kspace_solver.py:663); _Y = 3, _Z = 4 */
+#define KW_F_BONA 5     /* B/A */
+#define KW_F_TAU 6      /* absorption coefficient tau */
+#define KW_F_ETA 7      /* dispersion coefficient eta */
+#define KW_F_P0 8       /* initial pressure (already smoothed), kspace_solver.py:679-689 */
+#define KW_F_P 9        /* read: current pressure */
+#define KW_F_UX 10      /* read: current staggered velocity x (_Y 11, _Z 12) */
+#define KW_F_RHOX 13    /* read: split density x (_Y 14, _Z 15) */
+#define KW_F_REC_P 16   /* read: recorded p, [nrec][nsens] */
+#define KW_F_REC_UX 17  /* read: recorded staggered ux (uy 18, uz 19) */
+#define KW_F_AGG_P 20   /* read: running [max, min, sum of squares] of recorded p, [3][nsens] */
+#define KW_F_STEP 21    /* read: step counter (as float) */
+
+/* source modes (kspace_solver.py:60-114) */
+#define KW_SRC_DIRICHLET 1
+#define KW_SRC_ADDITIVE 2          /* k-space corrected: F^-1[kappa_s F[s]] */
+#define KW_SRC_ADDITIVE_NOCORR 4
+
+/* sensor record flags */
+#define KW_REC_P 1
+#define KW_REC_P_AGG 2
+#define KW_REC_UX 4  /* staggered velocity components: UY 8, UZ 16 */
+
+typedef struct kw_plan kw_plan;
+
+typedef struct {
+  int ndim;          /* 2 or 3 */
+  int n[3];          /* grid shape, C order (last axis contiguous) */
+  double spacing[3]; /* grid spacing per axis [m] */
+  int batch;         /* instances in this plan (same shape) */
+  int flags;         /* KW_NONLINEAR | KW_STOKES | KW_POWERLAW_* */
+  double alpha_power;/* y, used for the fractional Laplacian exponents */
+  int use_sg;        /* staggered grid shifts, kspace_solver.py:442-447 */
+  int device;        /* HIP device ordinal */
+} kw_plan_desc;
+
+typedef struct {
+  int nt;            /* number of time steps (kgrid.Nt) */
+  double dt;         /* time step (kgrid.dt) */
+  double c_ref;      /* max(c0), kspace_solver.py:231 */
+  int use_kspace;    /* kappa on/off, kspace_solver.py:429-434 */
+  const float* pml[3];    /* 1-D PML multipliers per axis (n[axis] values), kspace_solver.py:363-406 */
+  const float* pml_sg[3]; /* staggered-grid PML multipliers */
+} kw_inst_desc;
+
+/* Version of this ABI (KW_ABI_VERSION). */
+int kw_abi_version(void);
+/* Thread-local message describing the last failure. */
+const char* kw_last_error(void);
+/* Number of visible HIP devices. */
+int kw_device_count(int* count);
+/* 1 if an axis of this length has a compiled transform, else 0. */
+int kw_size_supported(int n);
+
+/* Replaces Simulation.__init__ + the grid part of setup(), kspace_solver.py:135-241. */
+int kw_plan_create(const kw_plan_desc* desc, kw_plan** out);
+/* Per-instance time step, c_ref, PML profiles; kspace_solver.py:225-231, 363-406. */
+int kw_plan_set_instance(kw_plan* plan, int b, const kw_inst_desc* d);
+/* Medium and initial-condition arrays, n = 1 (scalar) or the grid size;
+ * kspace_solver.py:228-230, 449-550, 647-689. */
+int kw_plan_set_field(kw_plan* plan, int b, int which, const float* data, int64_t n);
+/* Time-varying source term; which = -1 for p, 0..2 for ux/uy/uz.
+ * idx: C-order flat indices of the mask points (ascending), scale: per-point
+ * scale, sigrow: per-point row in sig ([nsigrow][len]), fac: per-axis
+ * multiplier (pressure sources are injected into each rho_i).
+ * Replaces _setup_source_operators + _build_source_op, kspace_solver.py:552-645, 60-114. */
+int kw_plan_set_source(kw_plan* plan, int b, int which, int mode, int64_t npts, const int64_t* idx, const float* scale,
+                       const int32_t* sigrow, const float* sig, int nsigrow, int len, const double fac[3]);
+/* Sensor: nsens points with ncorner (index, weight) pairs each (binary masks:
+ * ncorner = 1, weights NULL).  rec_start is 0-based.  Replaces
+ * _setup_sensor_mask / _setup_cartesian_extract, kspace_solver.py:243-361. */
+int kw_plan_set_sensor(kw_plan* plan, int b, int nsens, int ncorner, const int64_t* idx, const float* w, int rec_start,
+                       unsigned rec_flags);
+/* Zero the state and apply the t = 0 initial-value override where p0 is set
+ * (kspace_solver.py:651-654, 729-733); records step 0 for those instances. */
+int kw_plan_init(kw_plan* plan);
+/* Advance every active instance by up to nsteps steps (Simulation.step,
+ * kspace_solver.py:691-747); instances stop at their own Nt. */
+int kw_plan_run(kw_plan* plan, int64_t nsteps);
+/* Copy a device field of instance b to host memory (n floats). */
+int kw_plan_read(kw_plan* plan, int b, int which, float* out, int64_t n);
+/* Number of floats kw_plan_read returns for `which`. */
+int kw_plan_field_size(kw_plan* plan, int b, int which, int64_t* n);
+/* Release every device resource of the plan. */
+int kw_plan_destroy(kw_plan* plan);
+
+/* Spectral operator f -> real(ifftn(op * fftn(f))) on the device, the
+ * reference's Simulation._diff (kspace_solver.py:781-796), for a single field.
+ * kind: 0 identity, 1 gradient (op_grad), 2 divergence (op_div),
+ * 3 |k|^{power}, 5 source kappa, 6 unstagger;  kap = c_ref*dt/2 (0: kappa = 1).
+ * Used by the parity tests of the transform kernels. */
+int kw_spectral_apply(int ndim, const int n[3], const double spacing[3], int use_sg, double kap, int kind, int axis,
+                      double power, const float* f, float* out, int device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KWGPU_H */

@@ -1,0 +1,688 @@
+// C ABI of the solver: plan lifetime, host-side table construction (FP64 ->
+// FP32), step sequencing and hipGraph capture.  Declared in include/kwgpu.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kwgpu.h"
+#include "kw_launch.hpp"
+#include "kw_types.hpp"
+
+using namespace kw;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define KW_HIP(call)                                                                    \
+  do {                                                                                  \
+    hipError_t _e = (call);                                                             \
+    if (_e != hipSuccess) return fail(KW_EHIP, std::string(#call ": ") + hipGetErrorString(_e)); \
+  } while (0)
+
+struct DevBuf {
+  std::vector<void*> ptrs;
+  ~DevBuf() { release(); }
+  void release() {
+    for (void* p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+  }
+  template <class T>
+  int alloc(T** out, size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(e == hipErrorOutOfMemory ? KW_ENOMEM : KW_EHIP,
+                  std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+    }
+    ptrs.push_back(p);
+    *out = static_cast<T*>(p);
+    return KW_OK;
+  }
+  template <class T>
+  int upload(T** out, const T* host, size_t count) {
+    int rc = alloc(out, count);
+    if (rc) return rc;
+    if (count) KW_HIP(hipMemcpy(*out, host, count * sizeof(T), hipMemcpyHostToDevice));
+    return KW_OK;
+  }
+};
+
+// 1-D operator tables for one axis (FP64 construction, FP32 storage)
+struct AxisTables {
+  std::vector<float2> grad, div, unst, tw;
+  std::vector<float> k;
+};
+
+AxisTables build_axis(int n, double d, bool use_sg) {
+  AxisTables t;
+  t.grad.resize(n); t.div.resize(n); t.unst.resize(n); t.tw.resize(n); t.k.resize(n);
+  const double two_pi = 2.0 * M_PI;
+  for (int i = 0; i < n; ++i) {
+    // numpy.fft.fftfreq ordering: 0..ceil(n/2)-1, then -floor(n/2)..-1
+    const int f = (i < (n + 1) / 2) ? i : i - n;
+    const double k = two_pi * f / (n * d);
+    std::complex<double> ik(0.0, k);
+    std::complex<double> sp = use_sg ? std::exp(std::complex<double>(0.0, k * d / 2)) : 1.0;
+    std::complex<double> sm = use_sg ? std::exp(std::complex<double>(0.0, -k * d / 2)) : 1.0;
+    std::complex<double> g = ik * sp, dv = ik * sm, un = std::exp(std::complex<double>(0.0, -k * d / 2));
+    if (n % 2 == 0 && i == n / 2) {  // self-conjugate bin: keep the Hermitian part
+      g = g.real(); dv = dv.real(); un = un.real();
+    }
+    t.grad[i] = make_float2((float)g.real(), (float)g.imag());
+    t.div[i] = make_float2((float)dv.real(), (float)dv.imag());
+    t.unst[i] = make_float2((float)un.real(), (float)un.imag());
+    t.k[i] = (float)k;
+    const double a = -two_pi * i / n;
+    t.tw[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  return t;
+}
+
+Geom make_geom(int nd, const int* n) {
+  Geom g{};
+  g.nd = nd;
+  g.n[0] = n[0]; g.n[1] = n[1]; g.n[2] = nd == 3 ? n[2] : 1;
+  g.nz = n[nd - 1];
+  g.nh = g.nz / 2 + 1;
+  g.ntot = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  g.rows = (int)(g.ntot / g.nz);
+  g.nspec = (int64_t)g.rows * g.nh;
+  return g;
+}
+
+ColArgs col_args(const Geom& g, int axis, int mode) {
+  ColArgs a{};
+  a.mode = mode;
+  a.axis = axis;
+  a.ncols = g.nh;
+  if (g.nd == 2) {
+    a.estride = g.nh; a.ostride = 0; a.nouter = 1;
+  } else if (axis == 1) {
+    a.estride = g.nh; a.ostride = (int64_t)g.n[1] * g.nh; a.nouter = g.n[0];
+  } else {
+    a.estride = (int64_t)g.n[1] * g.nh; a.ostride = g.nh; a.nouter = g.n[1];
+  }
+  return a;
+}
+
+}  // namespace
+
+struct kw_plan {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  Geom g{};
+  int nb = 0;
+  int flags = 0;
+  double alpha_power = 1.5;
+  double spacing[3] = {0, 0, 0};
+  bool use_sg = true;
+  DevBuf mem;
+  Fields fl{};
+  Ops ops{};
+  std::vector<Inst> h_inst;
+  std::vector<char> inst_set;
+  std::vector<float*> p0;
+  Inst* d_inst = nullptr;
+  int nspecbuf = 0;       // allocated spectrum buffers
+  bool chain_p = false;
+  bool chain_u[3] = {false, false, false};
+  int max_sens = 0;
+  std::vector<int> nrec, nsens;
+  std::vector<unsigned> rec_flags;
+  hipGraphExec_t g1 = nullptr, gk = nullptr;
+  int graph_k = 8;
+  bool graphs = true;
+  bool inst_dirty = true;
+
+  ~kw_plan() {
+    if (g1) (void)hipGraphExecDestroy(g1);
+    if (gk) (void)hipGraphExecDestroy(gk);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+int sync_inst(kw_plan* P) {
+  if (!P->inst_dirty) return KW_OK;
+  KW_HIP(hipMemcpy(P->d_inst, P->h_inst.data(), sizeof(Inst) * P->nb, hipMemcpyHostToDevice));
+  P->inst_dirty = false;
+  return KW_OK;
+}
+
+int ensure_spec(kw_plan* P, int idx) {
+  while (P->nspecbuf <= idx) {
+    float2* s = nullptr;
+    int rc = P->mem.alloc(&s, (size_t)P->nb * P->g.nspec);
+    if (rc) return rc;
+    P->fl.spec[P->nspecbuf++] = s;
+  }
+  return KW_OK;
+}
+
+#define KW_LAUNCH(x)                                                              \
+  do {                                                                            \
+    hipError_t _e = (x);                                                          \
+    if (_e != hipSuccess) return fail(KW_EHIP, std::string(#x ": ") + hipGetErrorString(_e)); \
+  } while (0)
+
+// y-pass over a list of buffers (3-D only)
+int ypass(kw_plan* P, int mode, const std::vector<int>& bufs) {
+  if (P->g.nd != 3 || bufs.empty()) return KW_OK;
+  ColArgs a = col_args(P->g, 1, mode);
+  a.nbuf = (int)bufs.size();
+  for (size_t i = 0; i < bufs.size(); ++i) a.bufs[i] = bufs[i];
+  KW_LAUNCH(launch_col(P->g, P->nb, P->fl, P->ops, P->d_inst, a, P->g.n[1], P->st));
+  return KW_OK;
+}
+
+int xpass(kw_plan* P, const std::vector<XJob>& jobs) {
+  ColArgs a = col_args(P->g, 0, 2);
+  a.njob = (int)jobs.size();
+  for (size_t i = 0; i < jobs.size(); ++i) a.job[i] = jobs[i];
+  KW_LAUNCH(launch_col(P->g, P->nb, P->fl, P->ops, P->d_inst, a, P->g.n[0], P->st));
+  return KW_OK;
+}
+
+int enqueue_step(kw_plan* P) {
+  const Geom& g = P->g;
+  const int D = g.nd;
+  const float2* twz = P->ops.tw[D - 1];
+  int rc;
+  // --- momentum: grad p, u update, F_z[u_i] ---
+  std::vector<int> ub;
+  std::vector<XJob> jobs;
+  for (int i = 0; i < D; ++i) jobs.push_back({0, i, XK_GRAD, i});
+  for (int i = 0; i < D; ++i)
+    if (P->chain_u[i]) {
+      KW_LAUNCH(launch_src_build(g, P->nb, P->fl, P->d_inst, twz, i, P->st));
+      ub.push_back(D + 1 + i);
+      jobs.push_back({D + 1 + i, D + 1 + i, XK_SRCKAPPA, 0});
+    }
+  std::vector<int> y0 = {0};
+  y0.insert(y0.end(), ub.begin(), ub.end());
+  if ((rc = ypass(P, 0, y0))) return rc;
+  if ((rc = xpass(P, jobs))) return rc;
+  std::vector<int> y1;
+  for (int i = 0; i < D; ++i) y1.push_back(i);
+  y1.insert(y1.end(), ub.begin(), ub.end());
+  if ((rc = ypass(P, 1, y1))) return rc;
+  KW_LAUNCH(launch_momentum(g, P->nb, P->fl, P->d_inst, twz, P->st));
+  // --- mass: div u, rho update, EOS ---
+  jobs.clear();
+  std::vector<int> yb;
+  for (int i = 0; i < D; ++i) { jobs.push_back({i, i, XK_DIV, i}); yb.push_back(i); }
+  if (P->chain_p) {
+    KW_LAUNCH(launch_src_build(g, P->nb, P->fl, P->d_inst, twz, -1, P->st));
+    jobs.push_back({D, D, XK_SRCKAPPA, 0});
+    yb.push_back(D);
+  }
+  if ((rc = ypass(P, 0, yb))) return rc;
+  if ((rc = xpass(P, jobs))) return rc;
+  if ((rc = ypass(P, 1, yb))) return rc;
+  KW_LAUNCH(launch_mass(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st));
+  if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP)) {
+    jobs.clear();
+    yb.clear();
+    if (P->flags & KW_POWERLAW_ABS) { jobs.push_back({0, 0, XK_FRAC1, 0}); yb.push_back(0); }
+    if (P->flags & KW_POWERLAW_DISP) { jobs.push_back({1, 1, XK_FRAC2, 0}); yb.push_back(1); }
+    if ((rc = ypass(P, 0, yb))) return rc;
+    if ((rc = xpass(P, jobs))) return rc;
+    if ((rc = ypass(P, 1, yb))) return rc;
+    KW_LAUNCH(launch_eos(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st));
+  }
+  KW_LAUNCH(launch_gather(g, P->nb, P->max_sens, P->fl, P->d_inst, P->st));
+  KW_LAUNCH(launch_tick(P->fl, P->d_inst, P->nb, P->st));
+  return KW_OK;
+}
+
+int build_graphs(kw_plan* P) {
+  for (int pass = 0; pass < 2; ++pass) {
+    const int nsteps = pass == 0 ? 1 : P->graph_k;
+    KW_HIP(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
+    int rc = KW_OK;
+    for (int s = 0; s < nsteps && rc == KW_OK; ++s) rc = enqueue_step(P);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(P->st, &graph);
+    if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return fail(KW_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(KW_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    (pass == 0 ? P->g1 : P->gk) = ex;
+  }
+  return KW_OK;
+}
+
+bool valid_field_n(const kw_plan* P, int64_t n) { return n == 1 || n == P->g.ntot; }
+
+}  // namespace
+
+extern "C" {
+
+int kw_abi_version(void) { return KW_ABI_VERSION; }
+
+const char* kw_last_error(void) { return g_err.c_str(); }
+
+int kw_device_count(int* count) {
+  if (!count) return fail(KW_EINVAL, "count is NULL");
+  KW_HIP(hipGetDeviceCount(count));
+  return KW_OK;
+}
+
+int kw_size_supported(int n) { return size_supported(n) ? 1 : 0; }
+
+int kw_plan_create(const kw_plan_desc* d, kw_plan** out) {
+  if (!d || !out) return fail(KW_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (d->ndim != 2 && d->ndim != 3) return fail(KW_EUNSUPPORTED, "only 2-D and 3-D grids run on the GPU path");
+  if (d->batch < 1) return fail(KW_EINVAL, "batch must be >= 1");
+  for (int i = 0; i < d->ndim; ++i) {
+    if (d->n[i] < 2) return fail(KW_EINVAL, "every axis needs at least 2 points");
+    if (!size_supported(d->n[i]))
+      return fail(KW_EUNSUPPORTED, "axis length " + std::to_string(d->n[i]) + " has no compiled transform");
+    if (!(d->spacing[i] > 0)) return fail(KW_EINVAL, "grid spacing must be positive");
+  }
+  int ndev = 0;
+  KW_HIP(hipGetDeviceCount(&ndev));
+  if (d->device < 0 || d->device >= ndev) return fail(KW_EINVAL, "device ordinal out of range");
+  KW_HIP(hipSetDevice(d->device));
+  auto* P = new kw_plan();
+  P->dev = d->device;
+  P->g = make_geom(d->ndim, d->n);
+  P->nb = d->batch;
+  P->flags = d->flags;
+  P->alpha_power = d->alpha_power;
+  P->use_sg = d->use_sg != 0;
+  for (int i = 0; i < 3; ++i) P->spacing[i] = d->spacing[i];
+  P->graphs = std::getenv("KW_NO_GRAPH") == nullptr;
+  int rc = KW_OK;
+  auto cleanup = [&](int code) { delete P; return code; };
+  if (hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking) != hipSuccess) return cleanup(fail(KW_EHIP, "stream creation failed"));
+  const size_t nf = (size_t)P->nb * P->g.ntot;
+  if ((rc = P->mem.alloc(&P->fl.p, nf))) return cleanup(rc);
+  for (int i = 0; i < d->ndim; ++i) {
+    if ((rc = P->mem.alloc(&P->fl.u[i], nf))) return cleanup(rc);
+    if ((rc = P->mem.alloc(&P->fl.rho[i], nf))) return cleanup(rc);
+  }
+  if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP))
+    if ((rc = P->mem.alloc(&P->fl.rhot, nf))) return cleanup(rc);
+  if ((rc = ensure_spec(P, d->ndim - 1))) return cleanup(rc);
+  if ((rc = P->mem.alloc(&P->fl.tdev, P->nb))) return cleanup(rc);
+  if ((rc = P->mem.alloc(&P->d_inst, P->nb))) return cleanup(rc);
+  // operator tables
+  for (int ax = 0; ax < d->ndim; ++ax) {
+    AxisTables t = build_axis(d->n[ax], d->spacing[ax], P->use_sg);
+    float2 *gp, *dp, *up, *tp;
+    float* kp;
+    if ((rc = P->mem.upload(&gp, t.grad.data(), t.grad.size())) || (rc = P->mem.upload(&dp, t.div.data(), t.div.size())) ||
+        (rc = P->mem.upload(&up, t.unst.data(), t.unst.size())) || (rc = P->mem.upload(&tp, t.tw.data(), t.tw.size())) ||
+        (rc = P->mem.upload(&kp, t.k.data(), t.k.size())))
+      return cleanup(rc);
+    P->ops.grad[ax] = gp; P->ops.div[ax] = dp; P->ops.unst[ax] = up; P->ops.tw[ax] = tp; P->ops.k[ax] = kp;
+  }
+  P->ops.inv_ntot = (float)(1.0 / (double)P->g.ntot);
+  P->ops.frac1 = (float)(d->alpha_power - 2.0);
+  P->ops.frac2 = (float)(d->alpha_power - 1.0);
+  P->h_inst.assign(P->nb, Inst{});
+  P->inst_set.assign(P->nb, 0);
+  P->p0.assign(P->nb, nullptr);
+  P->nrec.assign(P->nb, 0);
+  P->nsens.assign(P->nb, 0);
+  P->rec_flags.assign(P->nb, 0);
+  *out = P;
+  return KW_OK;
+}
+
+int kw_plan_set_instance(kw_plan* P, int b, const kw_inst_desc* d) {
+  if (!P || !d || b < 0 || b >= P->nb) return fail(KW_EINVAL, "bad plan, instance or descriptor");
+  if (d->nt < 1 || !(d->dt > 0) || !(d->c_ref > 0)) return fail(KW_EINVAL, "nt, dt and c_ref must be positive");
+  KW_HIP(hipSetDevice(P->dev));
+  Inst& I = P->h_inst[b];
+  I.nt = d->nt;
+  I.dt = (float)d->dt;
+  I.kap = d->use_kspace ? (float)(d->c_ref * d->dt / 2.0) : 0.f;
+  for (int ax = 0; ax < P->g.nd; ++ax) {
+    if (!d->pml[ax] || !d->pml_sg[ax]) return fail(KW_EINVAL, "pml tables required for every axis");
+    float *a, *s;
+    int rc;
+    if ((rc = P->mem.upload(&a, d->pml[ax], P->g.n[ax])) || (rc = P->mem.upload(&s, d->pml_sg[ax], P->g.n[ax]))) return rc;
+    I.pml[ax] = a;
+    I.pmlsg[ax] = s;
+  }
+  P->inst_set[b] = 1;
+  P->inst_dirty = true;
+  return sync_inst(P);
+}
+
+int kw_plan_set_field(kw_plan* P, int b, int which, const float* data, int64_t n) {
+  if (!P || !data || b < 0 || b >= P->nb) return fail(KW_EINVAL, "bad plan, instance or data");
+  if (!valid_field_n(P, n)) return fail(KW_EINVAL, "field must have 1 or grid-size elements");
+  KW_HIP(hipSetDevice(P->dev));
+  Inst& I = P->h_inst[b];
+  float* dp;
+  int rc = P->mem.upload(&dp, data, (size_t)n);
+  if (rc) return rc;
+  const int s = n == 1 ? 0 : 1;
+  switch (which) {
+    case KW_F_C0SQ: I.c0sq = dp; I.c0sq_s = s; break;
+    case KW_F_RHO0: I.rho0 = dp; I.rho0_s = s; break;
+    case KW_F_DTR0_X: case KW_F_DTR0_X + 1: case KW_F_DTR0_X + 2: {
+      const int ax = which - KW_F_DTR0_X;
+      if (ax >= P->g.nd) return fail(KW_EINVAL, "dt/rho0 axis out of range");
+      if (I.dtr0_s != s && I.dtr0[0] && ax > 0) return fail(KW_EINVAL, "dt/rho0 arrays must all be scalar or all grid-sized");
+      I.dtr0[ax] = dp; I.dtr0_s = s; break;
+    }
+    case KW_F_BONA: I.bona = dp; I.bona_s = s; break;
+    case KW_F_TAU: I.tau = dp; I.tau_s = s; break;
+    case KW_F_ETA: I.eta = dp; I.eta_s = s; break;
+    case KW_F_P0:
+      if (n != P->g.ntot) return fail(KW_EINVAL, "p0 must be grid-sized");
+      P->p0[b] = dp; break;
+    default: return fail(KW_EINVAL, "unknown field id");
+  }
+  P->inst_dirty = true;
+  return sync_inst(P);
+}
+
+int kw_plan_set_source(kw_plan* P, int b, int which, int mode, int64_t npts, const int64_t* idx, const float* scale,
+                       const int32_t* sigrow, const float* sig, int nsigrow, int len, const double fac[3]) {
+  if (!P || b < 0 || b >= P->nb || which < -1 || which >= P->g.nd) return fail(KW_EINVAL, "bad plan, instance or source slot");
+  if (!P->inst_set[b]) return fail(KW_EINVAL, "kw_plan_set_instance must precede kw_plan_set_source");
+  if (npts < 1 || !idx || !scale || !sigrow || !sig || nsigrow < 1 || len < 1 || !fac)
+    return fail(KW_EINVAL, "source needs points, scales, signal rows and a signal");
+  if (mode != KW_SRC_DIRICHLET && mode != KW_SRC_ADDITIVE && mode != KW_SRC_ADDITIVE_NOCORR)
+    return fail(KW_EINVAL, "unknown source mode");
+  KW_HIP(hipSetDevice(P->dev));
+  const Geom& g = P->g;
+  for (int64_t i = 0; i < npts; ++i) {
+    if (idx[i] < 0 || idx[i] >= g.ntot || (i && idx[i] <= idx[i - 1])) return fail(KW_EINVAL, "source indices must be ascending and inside the grid");
+    if (sigrow[i] < 0 || sigrow[i] >= nsigrow) return fail(KW_EINVAL, "source signal row out of range");
+  }
+  Src s{};
+  s.len = len;
+  s.nsigrow = nsigrow;
+  for (int i = 0; i < 3; ++i) s.fac[i] = (float)fac[i];
+  int rc;
+  float* dsig;
+  if ((rc = P->mem.upload(&dsig, sig, (size_t)nsigrow * len))) return rc;
+  s.sig = dsig;
+  // CSR over grid rows
+  std::vector<int> row_ptr(g.rows + 1, 0), col(npts);
+  for (int64_t i = 0; i < npts; ++i) {
+    row_ptr[idx[i] / g.nz + 1]++;
+    col[i] = (int)(idx[i] % g.nz);
+  }
+  for (int r = 0; r < g.rows; ++r) row_ptr[r + 1] += row_ptr[r];
+  int *drp, *dcol, *dsr;
+  float* dsc;
+  if ((rc = P->mem.upload(&drp, row_ptr.data(), row_ptr.size())) || (rc = P->mem.upload(&dcol, col.data(), col.size())) ||
+      (rc = P->mem.upload(&dsc, scale, (size_t)npts)) || (rc = P->mem.upload(&dsr, sigrow, (size_t)npts)))
+    return rc;
+  s.row_ptr = drp; s.col = dcol; s.scale = dsc; s.sigrow = dsr;
+  if (mode == KW_SRC_DIRICHLET) s.mode = SRC_DIRICHLET;
+  else if (mode == KW_SRC_ADDITIVE_NOCORR) s.mode = SRC_NOCORR;
+  else if (nsigrow == 1) {
+    // single signal row: the filtered source is sig(t) * F^-1[kappa_s F[scale*mask]]
+    s.mode = SRC_PATTERN;
+    std::vector<float> dense((size_t)g.ntot, 0.f);
+    for (int64_t i = 0; i < npts; ++i) dense[idx[i]] = scale[i];
+    float* pat;
+    if ((rc = P->mem.upload(&pat, dense.data(), dense.size()))) return rc;
+    float2* tmp = nullptr;
+    if (hipMalloc(&tmp, sizeof(float2) * g.nspec) != hipSuccess) return fail(KW_ENOMEM, "pattern scratch allocation failed");
+    Fields f{};
+    f.spec[0] = tmp;
+    const Inst* di = P->d_inst + b;
+    const float2* twz = P->ops.tw[g.nd - 1];
+    hipError_t e = launch_zfwd(g, 1, pat, 0, tmp, 0, twz, P->st);
+    if (e == hipSuccess && g.nd == 3) {
+      ColArgs a = col_args(g, 1, 0);
+      a.nbuf = 1; a.bufs[0] = 0;
+      e = launch_col(g, 1, f, P->ops, di, a, g.n[1], P->st);
+    }
+    if (e == hipSuccess) {
+      ColArgs a = col_args(g, 0, 2);
+      a.njob = 1; a.job[0] = {0, 0, XK_SRCKAPPA, 0};
+      e = launch_col(g, 1, f, P->ops, di, a, g.n[0], P->st);
+    }
+    if (e == hipSuccess && g.nd == 3) {
+      ColArgs a = col_args(g, 1, 1);
+      a.nbuf = 1; a.bufs[0] = 0;
+      e = launch_col(g, 1, f, P->ops, di, a, g.n[1], P->st);
+    }
+    if (e == hipSuccess) e = launch_zinv(g, 1, tmp, 0, pat, 0, twz, 1.f, P->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(P->st);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(KW_EHIP, std::string("source pattern: ") + hipGetErrorString(e));
+    s.pattern = pat;
+  } else {
+    s.mode = SRC_CHAIN;
+    s.chain_buf = which < 0 ? g.nd : g.nd + 1 + which;
+    if ((rc = ensure_spec(P, s.chain_buf))) return rc;
+    if (which < 0) P->chain_p = true; else P->chain_u[which] = true;
+    if (P->g1) { (void)hipGraphExecDestroy(P->g1); P->g1 = nullptr; }
+    if (P->gk) { (void)hipGraphExecDestroy(P->gk); P->gk = nullptr; }
+  }
+  if (which < 0) P->h_inst[b].srcp = s; else P->h_inst[b].srcu[which] = s;
+  P->inst_dirty = true;
+  return sync_inst(P);
+}
+
+int kw_plan_set_sensor(kw_plan* P, int b, int nsens, int ncorner, const int64_t* idx, const float* w, int rec_start, unsigned rec_flags) {
+  if (!P || b < 0 || b >= P->nb) return fail(KW_EINVAL, "bad plan or instance");
+  if (!P->inst_set[b]) return fail(KW_EINVAL, "kw_plan_set_instance must precede kw_plan_set_sensor");
+  Inst& I = P->h_inst[b];
+  if (nsens < 1 || ncorner < 1 || !idx) return fail(KW_EINVAL, "sensor needs points");
+  if (rec_start < 0 || rec_start >= I.nt) return fail(KW_EINVAL, "record start outside [0, Nt)");
+  for (int64_t i = 0; i < (int64_t)nsens * ncorner; ++i)
+    if (idx[i] < 0 || idx[i] >= P->g.ntot) return fail(KW_EINVAL, "sensor index outside the grid");
+  KW_HIP(hipSetDevice(P->dev));
+  int rc;
+  int64_t* di;
+  if ((rc = P->mem.upload(&di, idx, (size_t)nsens * ncorner))) return rc;
+  I.sidx = di;
+  I.sens_kind = 1;
+  if (w) {
+    float* dw;
+    if ((rc = P->mem.upload(&dw, w, (size_t)nsens * ncorner))) return rc;
+    I.sw = dw;
+    I.sens_kind = 2;
+  }
+  I.nsens = nsens;
+  I.ncorner = ncorner;
+  I.rec_start = rec_start;
+  const int nrec = I.nt - rec_start;
+  I.rec_p = nullptr; I.agg_p = nullptr;
+  for (int i = 0; i < 3; ++i) I.rec_u[i] = nullptr;
+  if (rec_flags & KW_REC_P)
+    if ((rc = P->mem.alloc(&I.rec_p, (size_t)nrec * nsens))) return rc;
+  if (rec_flags & KW_REC_P_AGG)
+    if ((rc = P->mem.alloc(&I.agg_p, (size_t)3 * nsens))) return rc;
+  for (int i = 0; i < P->g.nd; ++i)
+    if (rec_flags & (KW_REC_UX << i))
+      if ((rc = P->mem.alloc(&I.rec_u[i], (size_t)nrec * nsens))) return rc;
+  P->nrec[b] = nrec;
+  P->nsens[b] = nsens;
+  P->rec_flags[b] = rec_flags;
+  if (nsens > P->max_sens) {
+    P->max_sens = nsens;
+    if (P->g1) { (void)hipGraphExecDestroy(P->g1); P->g1 = nullptr; }
+    if (P->gk) { (void)hipGraphExecDestroy(P->gk); P->gk = nullptr; }
+  }
+  P->inst_dirty = true;
+  return sync_inst(P);
+}
+
+int kw_plan_init(kw_plan* P) {
+  if (!P) return fail(KW_EINVAL, "NULL plan");
+  KW_HIP(hipSetDevice(P->dev));
+  const Geom& g = P->g;
+  int with_p0 = 0;
+  for (int b = 0; b < P->nb; ++b) {
+    const Inst& I = P->h_inst[b];
+    if (!P->inst_set[b]) return fail(KW_EINVAL, "instance " + std::to_string(b) + " not set");
+    if (!I.c0sq || !I.rho0) return fail(KW_EINVAL, "c0^2 and rho0 must be set");
+    for (int ax = 0; ax < g.nd; ++ax)
+      if (!I.dtr0[ax]) return fail(KW_EINVAL, "dt/rho0_sg must be set for every axis");
+    if ((P->flags & KW_NONLINEAR) && !I.bona) return fail(KW_EINVAL, "B/A must be set for a nonlinear plan");
+    if ((P->flags & (KW_STOKES | KW_POWERLAW_ABS)) && !I.tau) return fail(KW_EINVAL, "tau must be set for an absorbing plan");
+    if ((P->flags & KW_POWERLAW_DISP) && !I.eta) return fail(KW_EINVAL, "eta must be set for a dispersive plan");
+    with_p0 += P->p0[b] != nullptr;
+  }
+  if (with_p0 != 0 && with_p0 != P->nb) return fail(KW_EINVAL, "either every instance of a plan has p0 or none has");
+  int rc;
+  if ((rc = sync_inst(P))) return rc;
+  const int64_t nf = (int64_t)P->nb * g.ntot;
+  KW_LAUNCH(launch_fill(P->fl.p, nf, 0.f, P->st));
+  for (int i = 0; i < g.nd; ++i) {
+    KW_LAUNCH(launch_fill(P->fl.u[i], nf, 0.f, P->st));
+    KW_LAUNCH(launch_fill(P->fl.rho[i], nf, 0.f, P->st));
+  }
+  if (P->fl.rhot) KW_LAUNCH(launch_fill(P->fl.rhot, nf, 0.f, P->st));
+  for (int s = 0; s < P->nspecbuf; ++s) KW_LAUNCH(launch_fill(reinterpret_cast<float*>(P->fl.spec[s]), 2 * (int64_t)P->nb * g.nspec, 0.f, P->st));
+  KW_LAUNCH(launch_setint(P->fl.tdev, P->nb, 0, P->st));
+  if (with_p0) {
+    for (int b = 0; b < P->nb; ++b)
+      KW_HIP(hipMemcpyAsync(P->fl.p + (int64_t)b * g.ntot, P->p0[b], sizeof(float) * g.ntot, hipMemcpyDeviceToDevice, P->st));
+    const float2* twz = P->ops.tw[g.nd - 1];
+    KW_LAUNCH(launch_zfwd(g, P->nb, P->fl.p, g.ntot, P->fl.spec[0], g.nspec, twz, P->st));
+    if ((rc = ypass(P, 0, {0}))) return rc;
+    std::vector<XJob> jobs;
+    std::vector<int> yb;
+    for (int i = 0; i < g.nd; ++i) { jobs.push_back({0, i, XK_GRAD, i}); yb.push_back(i); }
+    if ((rc = xpass(P, jobs))) return rc;
+    if ((rc = ypass(P, 1, yb))) return rc;
+    KW_LAUNCH(launch_init_u(g, P->nb, P->fl, P->d_inst, twz, P->st));
+    KW_LAUNCH(launch_zfwd(g, P->nb, P->fl.p, g.ntot, P->fl.spec[0], g.nspec, twz, P->st));
+    KW_LAUNCH(launch_gather(g, P->nb, P->max_sens, P->fl, P->d_inst, P->st));
+    KW_LAUNCH(launch_tick(P->fl, P->d_inst, P->nb, P->st));
+  }
+  KW_HIP(hipStreamSynchronize(P->st));
+  return KW_OK;
+}
+
+int kw_plan_run(kw_plan* P, int64_t nsteps) {
+  if (!P || nsteps < 0) return fail(KW_EINVAL, "bad plan or step count");
+  KW_HIP(hipSetDevice(P->dev));
+  int rc;
+  if ((rc = sync_inst(P))) return rc;
+  if (P->graphs) {
+    if (!P->g1 && (rc = build_graphs(P))) return rc;
+    int64_t s = 0;
+    for (; s + P->graph_k <= nsteps; s += P->graph_k) KW_HIP(hipGraphLaunch(P->gk, P->st));
+    for (; s < nsteps; ++s) KW_HIP(hipGraphLaunch(P->g1, P->st));
+  } else {
+    for (int64_t s = 0; s < nsteps; ++s)
+      if ((rc = enqueue_step(P))) return rc;
+  }
+  KW_HIP(hipStreamSynchronize(P->st));
+  return KW_OK;
+}
+
+int kw_plan_field_size(kw_plan* P, int b, int which, int64_t* n) {
+  if (!P || !n || b < 0 || b >= P->nb) return fail(KW_EINVAL, "bad plan or instance");
+  const Geom& g = P->g;
+  if (which == KW_F_P || (which >= KW_F_UX && which < KW_F_UX + g.nd) || (which >= KW_F_RHOX && which < KW_F_RHOX + g.nd)) *n = g.ntot;
+  else if (which == KW_F_REC_P || (which >= KW_F_REC_UX && which < KW_F_REC_UX + g.nd)) *n = (int64_t)P->nrec[b] * P->nsens[b];
+  else if (which == KW_F_AGG_P) *n = 3 * (int64_t)P->nsens[b];
+  else if (which == KW_F_STEP) *n = 1;
+  else return fail(KW_EINVAL, "field cannot be read");
+  return KW_OK;
+}
+
+int kw_plan_read(kw_plan* P, int b, int which, float* out, int64_t n) {
+  int64_t need = 0;
+  int rc = kw_plan_field_size(P, b, which, &need);
+  if (rc) return rc;
+  if (!out || n != need) return fail(KW_EINVAL, "output buffer size mismatch: need " + std::to_string(need));
+  KW_HIP(hipSetDevice(P->dev));
+  KW_HIP(hipStreamSynchronize(P->st));
+  const Geom& g = P->g;
+  const Inst& I = P->h_inst[b];
+  const float* src = nullptr;
+  const int64_t off = (int64_t)b * g.ntot;
+  if (which == KW_F_P) src = P->fl.p + off;
+  else if (which >= KW_F_UX && which < KW_F_UX + 3) src = P->fl.u[which - KW_F_UX] + off;
+  else if (which >= KW_F_RHOX && which < KW_F_RHOX + 3) src = P->fl.rho[which - KW_F_RHOX] + off;
+  else if (which == KW_F_REC_P) src = I.rec_p;
+  else if (which >= KW_F_REC_UX && which < KW_F_REC_UX + 3) src = I.rec_u[which - KW_F_REC_UX];
+  else if (which == KW_F_AGG_P) src = I.agg_p;
+  else if (which == KW_F_STEP) {
+    int t = 0;
+    KW_HIP(hipMemcpy(&t, P->fl.tdev + b, sizeof(int), hipMemcpyDeviceToHost));
+    out[0] = (float)t;
+    return KW_OK;
+  }
+  if (!src) return fail(KW_EINVAL, "field was not recorded");
+  KW_HIP(hipMemcpy(out, src, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return KW_OK;
+}
+
+int kw_plan_destroy(kw_plan* P) {
+  if (!P) return KW_OK;
+  (void)hipSetDevice(P->dev);
+  if (P->st) (void)hipStreamSynchronize(P->st);
+  delete P;
+  return KW_OK;
+}
+
+int kw_spectral_apply(int ndim, const int n[3], const double spacing[3], int use_sg, double kap, int kind, int axis, double power,
+                      const float* f, float* out, int device) {
+  if (!n || !spacing || !f || !out) return fail(KW_EINVAL, "NULL argument");
+  kw_plan_desc d{};
+  d.ndim = ndim;
+  for (int i = 0; i < 3; ++i) { d.n[i] = n[i]; d.spacing[i] = spacing[i]; }
+  d.batch = 1;
+  d.alpha_power = power + 2.0;  // frac1 = power
+  d.use_sg = use_sg;
+  d.device = device;
+  if (kind < 0 || kind > 6 || kind == 4 || axis < 0 || axis >= ndim) return fail(KW_EINVAL, "bad operator kind or axis");
+  kw_plan* P = nullptr;
+  int rc = kw_plan_create(&d, &P);
+  if (rc) return rc;
+  const Geom& g = P->g;
+  std::vector<float> ones(std::max(n[0], std::max(n[1], ndim == 3 ? n[2] : 1)), 1.f);
+  kw_inst_desc id{};
+  id.nt = 1; id.dt = 1.0; id.c_ref = 1.0; id.use_kspace = 1;
+  for (int i = 0; i < ndim; ++i) { id.pml[i] = ones.data(); id.pml_sg[i] = ones.data(); }
+  if ((rc = kw_plan_set_instance(P, 0, &id))) { kw_plan_destroy(P); return rc; }
+  P->h_inst[0].kap = (float)kap;
+  P->inst_dirty = true;
+  auto run = [&]() -> int {
+    int r;
+    if ((r = sync_inst(P))) return r;
+    float* fin;
+    if ((r = P->mem.upload(&fin, f, (size_t)g.ntot))) return r;
+    KW_LAUNCH(launch_setint(P->fl.tdev, 1, 0, P->st));
+    const float2* twz = P->ops.tw[g.nd - 1];
+    KW_LAUNCH(launch_zfwd(g, 1, fin, 0, P->fl.spec[0], 0, twz, P->st));
+    if ((r = ypass(P, 0, {0}))) return r;
+    static const int kinds[7] = {XK_COPY, XK_GRAD, XK_DIV, XK_FRAC1, XK_FRAC2, XK_SRCKAPPA, XK_UNSTAGGER};
+    if ((r = xpass(P, {{0, 0, kinds[kind], axis}}))) return r;
+    if ((r = ypass(P, 1, {0}))) return r;
+    KW_LAUNCH(launch_zinv(g, 1, P->fl.spec[0], 0, fin, 0, twz, 1.f, P->st));
+    KW_HIP(hipStreamSynchronize(P->st));
+    KW_HIP(hipMemcpy(out, fin, sizeof(float) * g.ntot, hipMemcpyDeviceToHost));
+    return KW_OK;
+  };
+  rc = run();
+  kw_plan_destroy(P);
+  return rc;
+}
+
+}  // extern "C"

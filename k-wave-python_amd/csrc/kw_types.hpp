@@ -1,0 +1,107 @@
+// Device-visible parameter blocks shared by the kernels and the host plan code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace kw {
+
+constexpr int kMaxJobs = 8;
+constexpr int kMaxBufs = 8;
+
+// Geometry of one plan (every instance in a plan has the same grid shape).
+// Real fields are C-ordered with the last axis contiguous; spectra hold the
+// half spectrum along that axis:  spec[row * nh + k],  k < nh = nz/2 + 1.
+struct Geom {
+  int nd;            // 2 or 3
+  int n[3];          // axis lengths (n[2] = 1 in 2-D)
+  int nz;            // contiguous-axis length n[nd-1]
+  int nh;            // nz / 2 + 1
+  int rows;          // ntot / nz
+  int64_t ntot;      // real elements per instance
+  int64_t nspec;     // rows * nh complex elements per instance
+};
+
+// One time-varying source term (p, or one velocity component).
+struct Src {
+  int mode;            // SRC_* below
+  int len;             // signal length (injection stops once t >= len)
+  int nsigrow;         // rows in sig
+  const float* sig;    // [nsigrow][len]
+  const int* row_ptr;  // [rows + 1] CSR over grid rows (modes DIRICHLET, NOCORR, CHAIN)
+  const int* col;      // position along the contiguous axis
+  const float* scale;  // per-entry scale
+  const int* sigrow;   // per-entry signal row
+  const float* pattern;  // mode PATTERN: F^-1[kappa_s F[scale*mask]] (dense, ntot)
+  int chain_buf;       // mode CHAIN: spectrum buffer index holding the filtered source
+  float fac[3];        // per-axis multiplier applied on injection
+};
+enum { SRC_NONE = 0, SRC_DIRICHLET = 1, SRC_PATTERN = 2, SRC_CHAIN = 3, SRC_NOCORR = 4 };
+
+// Per-instance scalars, operator tables and medium arrays (stride 0 = scalar).
+struct Inst {
+  int nt;
+  float dt;
+  float kap;            // c_ref * dt / 2 (kappa = sinc(kap |k|)); use_kspace=False -> 0
+  const float* pml[3];  // 1-D multipliers along each axis (pressure grid)
+  const float* pmlsg[3];  // staggered grid
+  const float* c0sq; int c0sq_s;
+  const float* rho0; int rho0_s;
+  const float* dtr0[3]; int dtr0_s;  // dt / rho0_staggered_i
+  const float* bona; int bona_s;
+  const float* tau; int tau_s;       // absorption coefficient (grid)
+  const float* eta; int eta_s;       // dispersion coefficient (grid)
+  Src srcp;
+  Src srcu[3];
+  // sensors
+  int sens_kind;        // 0 none, 1 gather, 2 gather + linear weights
+  int nsens;
+  int ncorner;          // 1 (binary mask) or 2^nd (linear interpolation)
+  const int64_t* sidx;  // [nsens * ncorner]
+  const float* sw;      // [nsens * ncorner] (kind 2)
+  int rec_start;        // first recorded step (0-based)
+  float* rec_p;         // [nrec][nsens] or null
+  float* rec_u[3];      // staggered velocity, [nrec][nsens] or null
+  float* agg_p;         // [3][nsens]: running max, min, sum of squares (or null)
+};
+
+// Spectral operator tables shared by all instances of a plan.
+struct Ops {
+  const float2* grad[3];   // i k e^{+ik d/2}   (use_sg) or i k; real part only at Nyquist
+  const float2* div[3];    // i k e^{-ik d/2}   (use_sg) or i k
+  const float2* unst[3];   // e^{-ik d/2}, real part at Nyquist
+  const float* k[3];       // wavenumbers 2 pi fftfreq(n, d)
+  float inv_ntot;
+  float frac1, frac2;      // exponents y-2 and y-1 of the fractional Laplacians
+  const float2* tw[3];     // twiddle tables exp(-2 pi i m / n_axis)
+};
+
+enum { FLAG_NONLINEAR = 1, FLAG_STOKES = 2, FLAG_POWERLAW_ABS = 4, FLAG_POWERLAW_DISP = 8 };
+
+// Strided column pass (axes other than the contiguous one).
+enum { XK_COPY = 0, XK_GRAD = 1, XK_DIV = 2, XK_FRAC1 = 3, XK_FRAC2 = 4, XK_SRCKAPPA = 5, XK_UNSTAGGER = 6 };
+struct XJob {
+  int in, out, kind, axis;
+};
+struct ColArgs {
+  int mode;        // 0 forward, 1 inverse, 2 forward + jobs + inverse
+  int axis;        // axis transformed (0 or 1)
+  int nbuf;        // buffers transformed (modes 0/1)
+  int bufs[kMaxBufs];
+  int njob;
+  XJob job[kMaxJobs];
+  int64_t estride;  // element stride along the line
+  int64_t ostride;  // stride between consecutive "outer" indices
+  int nouter;       // outer index count (3-D y-pass: nx; x-pass: ny; 2-D: 1)
+  int ncols;        // columns (= nh)
+};
+
+struct Fields {
+  float* p;
+  float* u[3];
+  float* rho[3];
+  float* rhot;
+  float2* spec[kMaxBufs];
+  int* tdev;        // per-instance step counter
+};
+
+}  // namespace kw
