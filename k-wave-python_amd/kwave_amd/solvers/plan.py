@@ -1,0 +1,142 @@
+"""Owning wrapper of a device plan (a batch of same-shape solver instances)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .. import _abi
+from .._abi import check, f32, fptr
+
+
+def _i64(a):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+class DevicePlan:
+    """A batch of instances that share grid shape, spacing and physics flags."""
+
+    def __init__(self, setups, device=0):
+        s0 = setups[0]
+        for s in setups[1:]:
+            if s.shape != s0.shape or list(s.spacing) != list(s0.spacing):
+                raise ValueError("all instances of a plan need the same grid shape and spacing")
+            if s.flags != s0.flags or s.use_sg != s0.use_sg:
+                raise ValueError("all instances of a plan need the same physics switches")
+            if (s.p0 is None) != (s0.p0 is None):
+                raise ValueError("either every instance of a plan has p0 or none has")
+        L = _abi.lib()
+        d = _abi.PlanDesc()
+        d.ndim = s0.ndim
+        for i in range(3):
+            d.n[i] = s0.shape[i] if i < s0.ndim else 1
+            d.spacing[i] = s0.spacing[i] if i < s0.ndim else 1.0
+        d.batch = len(setups)
+        d.flags = s0.flags
+        d.alpha_power = s0.alpha_power
+        d.use_sg = int(s0.use_sg)
+        d.device = int(device)
+        self._h = C.c_void_p()
+        check(L.kw_plan_create(C.byref(d), C.byref(self._h)))
+        self.setups = setups
+        self.ndim = s0.ndim
+        self.shape = s0.shape
+        self._rec_flags = []
+        try:
+            for b, s in enumerate(setups):
+                self._upload(b, s)
+            check(L.kw_plan_init(self._h))
+        except Exception:
+            self.close()
+            raise
+
+    # -- upload -------------------------------------------------------------
+    def _field(self, b, which, val):
+        arr = f32(np.atleast_1d(val)).ravel()
+        check(_abi.lib().kw_plan_set_field(self._h, b, which, fptr(arr), arr.size))
+
+    def _upload(self, b, s):
+        L = _abi.lib()
+        inst = _abi.InstDesc()
+        inst.nt, inst.dt, inst.c_ref, inst.use_kspace = s.Nt, s.dt, s.c_ref, int(s.use_kspace)
+        keep = []
+        for ax in range(s.ndim):
+            a, g = f32(s.pml[ax]), f32(s.pml_sg[ax])
+            keep += [a, g]
+            inst.pml[ax] = fptr(a)
+            inst.pml_sg[ax] = fptr(g)
+        check(L.kw_plan_set_instance(self._h, b, C.byref(inst)))
+        self._field(b, _abi.F_C0SQ, np.asarray(s.c0) ** 2)
+        self._field(b, _abi.F_RHO0, s.rho0)
+        for ax in range(s.ndim):
+            self._field(b, _abi.F_DTR0_X + ax, s.dtr0[ax])
+        if s.bona is not None:
+            self._field(b, _abi.F_BONA, s.bona)
+        if s.tau is not None:
+            self._field(b, _abi.F_TAU, s.tau)
+        if s.eta is not None:
+            self._field(b, _abi.F_ETA, s.eta)
+        if s.p0 is not None:
+            self._field(b, _abi.F_P0, s.p0)
+        for which, term in [(-1, s.src_p)] + list(enumerate(s.src_u)):
+            if term is None:
+                continue
+            idx, ip = _i64(term.idx)
+            sc = f32(term.scale)
+            sr = np.ascontiguousarray(term.sigrow, dtype=np.int32)
+            sig = f32(term.sig)
+            fac = (C.c_double * 3)(*term.fac[:3])
+            check(L.kw_plan_set_source(self._h, b, which, term.mode, idx.size, ip, fptr(sc),
+                                       sr.ctypes.data_as(C.POINTER(C.c_int32)), fptr(sig), sig.shape[0], sig.shape[1], fac))
+        rec = s.record
+        flags = 0
+        want_series = "p" in rec
+        if want_series:
+            flags |= _abi.REC_P
+        for i, a in enumerate("xyz"[: s.ndim]):
+            if f"u{a}_staggered" in rec:
+                flags |= _abi.REC_UX << i
+        unsupported = [k for k in rec if (k.startswith("u") and not k.endswith("_staggered") and not k.endswith("_final")) or k.startswith("I")]
+        if unsupported:
+            raise NotImplementedError(f"recording {sorted(unsupported)} is not available on the GPU path yet")
+        self._rec_flags.append(flags)
+        if flags:
+            sidx, sp = _i64(s.sensor_idx)
+            w = f32(s.sensor_w) if s.sensor_w is not None else None
+            check(L.kw_plan_set_sensor(self._h, b, s.n_sensor, sidx.shape[1], sp, fptr(w) if w is not None else None,
+                                       s.rec_start, flags))
+
+    # -- run / read ---------------------------------------------------------
+    def run(self, nsteps):
+        check(_abi.lib().kw_plan_run(self._h, int(nsteps)))
+
+    def read(self, b, which):
+        n = C.c_int64(0)
+        L = _abi.lib()
+        check(L.kw_plan_field_size(self._h, b, which, C.byref(n)))
+        out = np.empty(n.value, dtype=np.float32)
+        check(L.kw_plan_read(self._h, b, which, fptr(out), out.size))
+        return out
+
+    def step_index(self, b=0):
+        return int(self.read(b, _abi.F_STEP)[0])
+
+    def recorded(self, b, which):
+        """Recorded series as (n_sensor, n_recorded_steps)."""
+        s = self.setups[b]
+        return self.read(b, which).reshape(s.Nt - s.rec_start, s.n_sensor).T
+
+    def field(self, b, which):
+        return self.read(b, which).reshape(self.shape)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _abi.lib().kw_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

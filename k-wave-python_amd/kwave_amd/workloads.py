@@ -1,0 +1,145 @@
+"""Seeded synthetic inputs for the benchmark and parity workloads.
+
+Each builder returns ``(kgrid, medium, source, sensor, pml_size, pml_alpha)``
+ready for ``kspaceFirstOrder2D/3D`` (PML inside the grid).  Every random
+choice goes through ``numpy.random.default_rng(seed)``; the definitions are
+frozen here so results are comparable across runs.
+
+* ``ivp_2d``      128^2 homogeneous lossless initial-value problem, 50-point
+                  Cartesian circle sensor (Nt = 664).
+* ``hetero_2d``   512^2 heterogeneous absorbing medium (8 smoothed elliptical
+                  inclusions), 64-cell additive line source (1 MHz, 3-cycle
+                  Hann tone burst, 1 MPa), 256-cell line sensor, 2000 steps.
+* ``nonlinear_3d`` 512^3 Gaussian-random c/rho, B/A = 6, power-law absorption,
+                  16x16 planar source (0.5 MHz, 4 cycles, 2 MPa), 256x256
+                  planar sensor recording p and p_max, 1500 steps.
+Each takes a ``scale`` argument that shrinks the grid and step count
+proportionally for parity tests.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .data import Vector
+from .kgrid import kWaveGrid
+from .kmedium import kWaveMedium
+from .ksensor import kSensor
+from .ksource import kSource
+from .utils.mapgen import make_cart_circle, make_disc
+from .utils.signals import get_win, tone_burst
+
+
+def _gaussian_filter_periodic(a, sigma):
+    """Periodic convolution with a unit-area Gaussian of std ``sigma`` cells (spectral)."""
+    import scipy.fft as sfft
+
+    spec = sfft.rfftn(a, workers=-1)
+    shape = a.shape
+    k2 = 0
+    for ax, n in enumerate(shape):
+        f = sfft.rfftfreq(n) if ax == len(shape) - 1 else sfft.fftfreq(n)
+        sh = [1] * len(shape)
+        sh[ax] = f.size
+        k2 = k2 + ((2 * np.pi * f) ** 2).reshape(sh)
+    spec *= np.exp(-0.5 * sigma ** 2 * k2)
+    return sfft.irfftn(spec, s=shape, workers=-1)
+
+
+def _affine(a, lo, hi):
+    mn, mx = a.min(), a.max()
+    return lo + (a - mn) * (hi - lo) / (mx - mn)
+
+
+def ivp_2d(seed=1, n=128, pml=20):
+    """128^2 homogeneous IVP: two smoothed discs, Cartesian circle sensor."""
+    rng = np.random.default_rng(seed)
+    dx = 1e-4
+    kgrid = kWaveGrid(Vector([n, n]), Vector([dx, dx]))
+    c = 1500.0
+    dt = 0.3 * dx / c
+    t_end = 1.1 * np.sqrt(2) * n * dx / c
+    kgrid.setTime(int(np.floor(t_end / dt) + 1), dt)
+    medium = kWaveMedium(sound_speed=c, density=1000.0)
+    p0 = np.zeros((n, n))
+    for r, amp in ((5, 5.0), (3, 3.0)):
+        cx, cy = rng.integers(pml + r + 3, n - pml - r - 2, size=2)
+        p0 = p0 + amp * make_disc(Vector([n, n]), Vector([int(cx), int(cy)]), r)
+    source = kSource()
+    source.p0 = p0
+    sensor = kSensor(make_cart_circle(4e-3 * n / 128, 50))
+    sensor.record = ["p"]
+    return kgrid, medium, source, sensor, (pml, pml), (2.0, 2.0)
+
+
+def hetero_2d(seed=2, n=512, nt=2000, pml=20):
+    """512^2 heterogeneous absorbing medium with a time-varying line source."""
+    rng = np.random.default_rng(seed)
+    s = n / 512
+    dx = 5e-5
+    c = np.full((n, n), 1500.0)
+    rho = np.full((n, n), 1000.0)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    lo, hi = pml + int(40 * s), n - pml - int(40 * s)
+    for _ in range(8):
+        cx, cy = rng.uniform(lo, hi, size=2)
+        a, b = rng.uniform(8 * s, 40 * s, size=2)
+        th = rng.uniform(0, np.pi)
+        cv, rv = rng.uniform(1400, 1800), rng.uniform(1000, 1200)
+        x, y = ii - cx, jj - cy
+        u = x * np.cos(th) + y * np.sin(th)
+        v = -x * np.sin(th) + y * np.cos(th)
+        inside = (u / a) ** 2 + (v / b) ** 2 <= 1
+        c[inside] = cv
+        rho[inside] = rv
+    from scipy.ndimage import gaussian_filter
+
+    c = gaussian_filter(c, 2 * max(s, 0.25), mode="reflect")
+    rho = gaussian_filter(rho, 2 * max(s, 0.25), mode="reflect")
+    kgrid = kWaveGrid(Vector([n, n]), Vector([dx, dx]))
+    dt = 0.3 * dx / c.max()
+    kgrid.setTime(int(nt), dt)
+    medium = kWaveMedium(sound_speed=c, density=rho, alpha_coeff=0.75, alpha_power=1.5)
+    f0 = 1e6
+    nsig = int(round(3 / (f0 * dt)))
+    t = np.arange(nsig) * dt
+    sig = 1e6 * np.sin(2 * np.pi * f0 * t) * np.squeeze(get_win(nsig, "Hanning")[0])
+    source = kSource()
+    mask = np.zeros((n, n), dtype=bool)
+    x0 = int(round(30 * s))
+    mask[x0, n // 2 - int(32 * s): n // 2 + int(32 * s)] = True
+    source.p_mask = mask
+    source.p = sig[None, :]
+    smask = np.zeros((n, n), dtype=bool)
+    smask[n - 1 - int(round(30 * s)), n // 4: 3 * n // 4] = True
+    sensor = kSensor(smask)
+    sensor.record = ["p"]
+    return kgrid, medium, source, sensor, (pml, pml), (2.0, 2.0)
+
+
+def nonlinear_3d(seed=3, n=512, nt=1500, pml=16):
+    """512^3 heterogeneous nonlinear absorbing medium, planar source and sensor."""
+    rng = np.random.default_rng(seed)
+    s = n / 512
+    dx = 1e-4
+    shape = (n, n, n)
+    sig_cells = 8 * s
+    g1 = _gaussian_filter_periodic(rng.standard_normal(shape, dtype=np.float32), sig_cells)
+    g2 = _gaussian_filter_periodic(rng.standard_normal(shape, dtype=np.float32), sig_cells)
+    c = _affine(g1.astype(np.float64), 1450.0, 1650.0)
+    rho = _affine(g2.astype(np.float64), 950.0, 1100.0)
+    del g1, g2
+    kgrid = kWaveGrid(Vector([n, n, n]), Vector([dx, dx, dx]))
+    dt = 0.3 * dx / 1650.0
+    kgrid.setTime(int(nt), dt)
+    medium = kWaveMedium(sound_speed=c, density=rho, alpha_coeff=0.5, alpha_power=1.1, BonA=6.0)
+    source = kSource()
+    mask = np.zeros(shape, dtype=bool)
+    h = max(int(8 * s), 1)
+    mask[int(round(24 * s)), n // 2 - h: n // 2 + h, n // 2 - h: n // 2 + h] = True
+    source.p_mask = mask
+    source.p = 2e6 * tone_burst(1 / dt, 0.5e6, 4)
+    smask = np.zeros(shape, dtype=bool)
+    smask[n // 2, n // 4: 3 * n // 4, n // 4: 3 * n // 4] = True
+    sensor = kSensor(smask)
+    sensor.record = ["p", "p_max"]
+    return kgrid, medium, source, sensor, (pml, pml, pml), (2.0, 2.0, 2.0)
