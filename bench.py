@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Benchmark: the 512^3 heterogeneous nonlinear absorbing k-space PSTD time loop on one MI355X.
+
+Workload (frozen in kwave_amd/workloads.py::nonlinear_3d): 512^3 grid with a
+16-cell PML, Gaussian-random c in [1450, 1650] m/s and rho in [950, 1100]
+kg/m^3, B/A = 6, power-law absorption 0.5 dB/(MHz^y cm) with y = 1.1, a 16x16
+planar additive source (0.5 MHz, 4 cycles, 2 MPa) and a 256x256 planar sensor
+recording p and p_max, 1500 steps, float32 arithmetic.
+
+A "step" is one time step of that simulation.  W warm-up steps run first, the
+state is re-initialised, then exactly K steps are timed between two device
+synchronisations, with HIP events bracketing every kernel launch on the plan
+stream (per-kernel device time, used for the roofline of the dominant kernel).
+
+Prints ONE JSON line on stdout.  Everything else goes to stderr.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "k-wave-python_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# fused-minimum algorithmic bytes per voxel-step of this workload: 412 B
+# (3-D transforms + fused pointwise, power-law, nonlinear) plus 60 B on the
+# source's ~440 active steps of 1500
+SURVEY_B_ALG = 412.0 + 60.0 * 440.0 / 1500.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(n=128, steps=5):
+    """FP64 oracle (numpy, single thread) on the same workload at n^3 for `steps` steps."""
+    sys.path.insert(0, str(ROOT))
+    from kwave_amd import workloads
+    from oracle.kspace_ref import OracleSimulation
+
+    kg, med, src, sen, pml, alpha = workloads.nonlinear_3d(seed=3, n=n, nt=steps + 1, pml=max(2, 16 * n // 512))
+    sim = OracleSimulation(kg, med, src, sen, pml_size=pml, pml_alpha=alpha)
+    sim.step()  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    dt = time.perf_counter() - t0
+    return {
+        "value": n ** 3 * steps / dt,
+        "unit": "voxel-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"FP64 numpy oracle, same workload at {n}^3 for {steps} steps after 1 warm-up step "
+                  f"({dt:.1f} s, 1 thread of {os.cpu_count()} on '{cpu_model()}')",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512, help="grid edge (512 is the benchmark workload)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        log("this benchmark runs on one GPU only: the 512^3 workload is a single simulation on one MI355X; "
+            "multi-GPU sharding (slab decomposition with all-to-all transposes) is a design note, not a build")
+        sys.exit(2)
+
+    from kwave_amd import workloads
+    from kwave_amd.solvers.plan import DevicePlan
+    from kwave_amd.solvers.setup import build_instance
+
+    n = args.n
+    nt_cfg = 1500
+    steps = min(args.steps, nt_cfg)
+    t0 = time.perf_counter()
+    kg, med, src, sen, pml, alpha = workloads.nonlinear_3d(seed=3, n=n, nt=nt_cfg, pml=16 * n // 512 or 2)
+    st = build_instance(kg, med, src, sen, pml_size=pml, pml_alpha=alpha)
+    del med
+    log(f"workload built in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    plan = DevicePlan([st])
+    log(f"device plan ready in {time.perf_counter() - t0:.1f} s")
+
+    if args.warmup:
+        plan.run(min(args.warmup, nt_cfg))
+        plan.reset()
+    import ctypes
+
+    from kwave_amd import _abi
+
+    _abi.check(_abi.lib().kw_plan_run(plan._h, 0))  # synchronise
+    t0 = time.perf_counter()
+    kern = plan.run_profiled(steps)
+    wall = time.perf_counter() - t0
+    log(f"timed {steps} steps in {wall:.3f} s")
+    ntot = n ** 3
+    rate = ntot * steps / wall
+    # dominant kernel
+    dom = max(kern, key=lambda k: kern[k][0])
+    ms, by, nl = kern[dom]
+    avg_ms = ms / nl
+    achieved = by / nl / (avg_ms * 1e-3) / 1e9
+    kernels = {k: {"launches": v[2], "avg_us": 1e3 * v[0] / v[2], "total_ms": v[0],
+                   "GBps": (v[1] / v[2]) / (v[0] / v[2] * 1e-3) / 1e9 if v[1] else None,
+                   "bytes_per_launch": v[1] / v[2]} for k, v in kern.items()}
+    alg_bytes_step = sum(v[1] for v in kern.values()) / steps
+    gpu_ms_step = sum(v[0] for v in kern.values()) / steps
+    p_rec = plan.read(0, _abi.F_REC_P)
+    finite = bool(abs(p_rec).max() < 1e12)
+    out = {
+        "metric": "voxel-steps/s, %HBM peak (512^3)",
+        "value": rate,
+        "unit": "voxel-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"3d_nonlinear_{n}", "grid": [n, n, n], "nt": nt_cfg, "pml": pml[0],
+                   "physics": "heterogeneous c/rho, B/A=6, power-law 0.5 dB/(MHz^1.1 cm)",
+                   "source": "16x16 planar additive", "sensor": "256x256 plane, p + p_max"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "hbm_frac_step_model": SURVEY_B_ALG * rate / (HBM_PEAK_GBS * 1e9),
+        "alg_bytes_per_voxel_step": alg_bytes_step / ntot,
+        "gpu_ms_per_step": gpu_ms_step,
+        "kernels": kernels,
+        "finite": finite,
+    }
+    plan.close()
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # the GPU figure stands without it
+            log(f"cpu baseline failed: {e!r}")
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

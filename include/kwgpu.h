@@ -122,6 +122,14 @@ int kw_plan_init(kw_plan* plan);
 /* Advance every active instance by up to nsteps steps (Simulation.step,
  * kspace_solver.py:691-747); instances stop at their own Nt. */
 int kw_plan_run(kw_plan* plan, int64_t nsteps);
+/* Like kw_plan_run, without graph capture, bracketing every kernel launch with
+ * HIP events on the plan stream.  Per kernel kind k (< kw_kernel_kind_count())
+ * it returns the summed device time [ms], the algorithmic bytes moved and the
+ * launch count.  No counterpart in the reference (its loop has no per-kernel
+ * timing); used by the benchmark to place each kernel against the HBM roofline. */
+int kw_plan_run_profiled(kw_plan* plan, int64_t nsteps, double* kernel_ms, double* kernel_bytes, int64_t* launches);
+int kw_kernel_kind_count(void);
+const char* kw_kernel_kind_name(int kind);
 /* Copy a device field of instance b to host memory (n floats). */
 int kw_plan_read(kw_plan* plan, int b, int which, float* out, int64_t n);
 /* Number of floats kw_plan_read returns for `which`. */

@@ -146,8 +146,15 @@ struct kw_plan {
   int graph_k = 8;
   bool graphs = true;
   bool inst_dirty = true;
+  // per-kernel timing (kw_plan_run_profiled)
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  struct Rec { int kind; size_t e0, e1; double bytes; };
+  std::vector<Rec> recs;
 
   ~kw_plan() {
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     if (g1) (void)hipGraphExecDestroy(g1);
     if (gk) (void)hipGraphExecDestroy(gk);
     if (st) (void)hipStreamDestroy(st);
@@ -179,22 +186,92 @@ int ensure_spec(kw_plan* P, int idx) {
     if (_e != hipSuccess) return fail(KW_EHIP, std::string(#x ": ") + hipGetErrorString(_e)); \
   } while (0)
 
+enum { KK_YPASS = 0, KK_XGRAD, KK_XDIV, KK_XFRAC, KK_MOMENTUM, KK_MASS, KK_EOS, KK_SRC, KK_GATHER, KK_TICK, KK_COUNT };
+const char* const kKindNames[KK_COUNT] = {"col_y", "col_x_grad", "col_x_div", "col_x_frac", "z_momentum",
+                                          "z_mass",  "z_eos",      "z_src",     "gather",     "tick"};
+
+int ev_get(kw_plan* P, size_t* idx) {
+  if (P->ev_used == P->ev_pool.size()) {
+    hipEvent_t e;
+    KW_HIP(hipEventCreate(&e));
+    P->ev_pool.push_back(e);
+  }
+  *idx = P->ev_used++;
+  return KW_OK;
+}
+
+// Launch `x`; in profiling mode bracket it with events on the plan stream and
+// remember the algorithmic bytes the launch moves.
+#define KW_TLAUNCH(kind, bytes, x)                                                 \
+  do {                                                                             \
+    size_t _i0 = 0, _i1 = 0;                                                       \
+    if (P->prof) {                                                                 \
+      int _r = ev_get(P, &_i0);                                                    \
+      if (_r) return _r;                                                           \
+      KW_HIP(hipEventRecord(P->ev_pool[_i0], P->st));                              \
+    }                                                                              \
+    KW_LAUNCH(x);                                                                  \
+    if (P->prof) {                                                                 \
+      int _r = ev_get(P, &_i1);                                                    \
+      if (_r) return _r;                                                           \
+      KW_HIP(hipEventRecord(P->ev_pool[_i1], P->st));                              \
+      P->recs.push_back({(kind), _i0, _i1, (double)(bytes)});                      \
+    }                                                                              \
+  } while (0)
+
 // y-pass over a list of buffers (3-D only)
 int ypass(kw_plan* P, int mode, const std::vector<int>& bufs) {
   if (P->g.nd != 3 || bufs.empty()) return KW_OK;
   ColArgs a = col_args(P->g, 1, mode);
   a.nbuf = (int)bufs.size();
   for (size_t i = 0; i < bufs.size(); ++i) a.bufs[i] = bufs[i];
-  KW_LAUNCH(launch_col(P->g, P->nb, P->fl, P->ops, P->d_inst, a, P->g.n[1], P->st));
+  const double spec_bytes = 8.0 * P->nb * (double)P->g.nspec;
+  KW_TLAUNCH(KK_YPASS, 2.0 * spec_bytes * bufs.size(), launch_col(P->g, P->nb, P->fl, P->ops, P->d_inst, a, P->g.n[1], P->st));
   return KW_OK;
 }
 
-int xpass(kw_plan* P, const std::vector<XJob>& jobs) {
+int xpass(kw_plan* P, const std::vector<XJob>& jobs, int kind) {
   ColArgs a = col_args(P->g, 0, 2);
   a.njob = (int)jobs.size();
-  for (size_t i = 0; i < jobs.size(); ++i) a.job[i] = jobs[i];
-  KW_LAUNCH(launch_col(P->g, P->nb, P->fl, P->ops, P->d_inst, a, P->g.n[0], P->st));
+  int nin = 0, last = -1;
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    a.job[i] = jobs[i];
+    if (jobs[i].in != last) { ++nin; last = jobs[i].in; }
+  }
+  const double spec_bytes = 8.0 * P->nb * (double)P->g.nspec;
+  KW_TLAUNCH(kind, spec_bytes * (nin + jobs.size()), launch_col(P->g, P->nb, P->fl, P->ops, P->d_inst, a, P->g.n[0], P->st));
   return KW_OK;
+}
+
+// algorithmic bytes of the z-pass kernels (state arrays, medium arrays, spectra)
+double zbytes_momentum(const kw_plan* P) {
+  const double nr = (double)P->nb * P->g.ntot, ns = (double)P->nb * P->g.nspec;
+  const double dtr = P->h_inst[0].dtr0_s ? 4.0 * nr : 0.0;
+  return P->g.nd * (16.0 * ns + 8.0 * nr + dtr);
+}
+double zbytes_mass(const kw_plan* P) {
+  const kw::Inst& I = P->h_inst[0];
+  const double nr = (double)P->nb * P->g.ntot, ns = (double)P->nb * P->g.nspec;
+  const int D = P->g.nd;
+  double b = 8.0 * ns * D + 8.0 * nr * D + (I.rho0_s ? 4.0 * nr : 0.0);
+  if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP)) {
+    b += 4.0 * nr + 8.0 * ns * (((P->flags & KW_POWERLAW_ABS) ? 1 : 0) + ((P->flags & KW_POWERLAW_DISP) ? 1 : 0));
+  } else {
+    b += 4.0 * nr + 8.0 * ns + (I.c0sq_s ? 4.0 * nr : 0.0);
+    if ((P->flags & KW_STOKES) && I.tau_s) b += 4.0 * nr;
+    if ((P->flags & KW_NONLINEAR) && I.bona_s) b += 4.0 * nr;
+  }
+  return b;
+}
+double zbytes_eos(const kw_plan* P) {
+  const kw::Inst& I = P->h_inst[0];
+  const double nr = (double)P->nb * P->g.ntot, ns = (double)P->nb * P->g.nspec;
+  double b = 4.0 * nr + 4.0 * nr + 8.0 * ns;  // rho_total, p, P_z
+  if (P->flags & KW_POWERLAW_ABS) b += 8.0 * ns + (I.tau_s ? 4.0 * nr : 0.0);
+  if (P->flags & KW_POWERLAW_DISP) b += 8.0 * ns + (I.eta_s ? 4.0 * nr : 0.0);
+  if (I.c0sq_s) b += 4.0 * nr;
+  if (P->flags & KW_NONLINEAR) b += (I.bona_s ? 4.0 * nr : 0.0) + (I.rho0_s ? 4.0 * nr : 0.0);
+  return b;
 }
 
 int enqueue_step(kw_plan* P) {
@@ -208,44 +285,44 @@ int enqueue_step(kw_plan* P) {
   for (int i = 0; i < D; ++i) jobs.push_back({0, i, XK_GRAD, i});
   for (int i = 0; i < D; ++i)
     if (P->chain_u[i]) {
-      KW_LAUNCH(launch_src_build(g, P->nb, P->fl, P->d_inst, twz, i, P->st));
+      KW_TLAUNCH(KK_SRC, 0.0, launch_src_build(g, P->nb, P->fl, P->d_inst, twz, i, P->st));
       ub.push_back(D + 1 + i);
       jobs.push_back({D + 1 + i, D + 1 + i, XK_SRCKAPPA, 0});
     }
   std::vector<int> y0 = {0};
   y0.insert(y0.end(), ub.begin(), ub.end());
   if ((rc = ypass(P, 0, y0))) return rc;
-  if ((rc = xpass(P, jobs))) return rc;
+  if ((rc = xpass(P, jobs, KK_XGRAD))) return rc;
   std::vector<int> y1;
   for (int i = 0; i < D; ++i) y1.push_back(i);
   y1.insert(y1.end(), ub.begin(), ub.end());
   if ((rc = ypass(P, 1, y1))) return rc;
-  KW_LAUNCH(launch_momentum(g, P->nb, P->fl, P->d_inst, twz, P->st));
+  KW_TLAUNCH(KK_MOMENTUM, zbytes_momentum(P), launch_momentum(g, P->nb, P->fl, P->d_inst, twz, P->st));
   // --- mass: div u, rho update, EOS ---
   jobs.clear();
   std::vector<int> yb;
   for (int i = 0; i < D; ++i) { jobs.push_back({i, i, XK_DIV, i}); yb.push_back(i); }
   if (P->chain_p) {
-    KW_LAUNCH(launch_src_build(g, P->nb, P->fl, P->d_inst, twz, -1, P->st));
+    KW_TLAUNCH(KK_SRC, 0.0, launch_src_build(g, P->nb, P->fl, P->d_inst, twz, -1, P->st));
     jobs.push_back({D, D, XK_SRCKAPPA, 0});
     yb.push_back(D);
   }
   if ((rc = ypass(P, 0, yb))) return rc;
-  if ((rc = xpass(P, jobs))) return rc;
+  if ((rc = xpass(P, jobs, KK_XDIV))) return rc;
   if ((rc = ypass(P, 1, yb))) return rc;
-  KW_LAUNCH(launch_mass(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st));
+  KW_TLAUNCH(KK_MASS, zbytes_mass(P), launch_mass(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st));
   if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP)) {
     jobs.clear();
     yb.clear();
     if (P->flags & KW_POWERLAW_ABS) { jobs.push_back({0, 0, XK_FRAC1, 0}); yb.push_back(0); }
     if (P->flags & KW_POWERLAW_DISP) { jobs.push_back({1, 1, XK_FRAC2, 0}); yb.push_back(1); }
     if ((rc = ypass(P, 0, yb))) return rc;
-    if ((rc = xpass(P, jobs))) return rc;
+    if ((rc = xpass(P, jobs, KK_XFRAC))) return rc;
     if ((rc = ypass(P, 1, yb))) return rc;
-    KW_LAUNCH(launch_eos(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st));
+    KW_TLAUNCH(KK_EOS, zbytes_eos(P), launch_eos(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st));
   }
-  KW_LAUNCH(launch_gather(g, P->nb, P->max_sens, P->fl, P->d_inst, P->st));
-  KW_LAUNCH(launch_tick(P->fl, P->d_inst, P->nb, P->st));
+  KW_TLAUNCH(KK_GATHER, 0.0, launch_gather(g, P->nb, P->max_sens, P->fl, P->d_inst, P->st));
+  KW_TLAUNCH(KK_TICK, 0.0, launch_tick(P->fl, P->d_inst, P->nb, P->st));
   return KW_OK;
 }
 
@@ -565,7 +642,7 @@ int kw_plan_init(kw_plan* P) {
     std::vector<XJob> jobs;
     std::vector<int> yb;
     for (int i = 0; i < g.nd; ++i) { jobs.push_back({0, i, XK_GRAD, i}); yb.push_back(i); }
-    if ((rc = xpass(P, jobs))) return rc;
+    if ((rc = xpass(P, jobs, KK_XGRAD))) return rc;
     if ((rc = ypass(P, 1, yb))) return rc;
     KW_LAUNCH(launch_init_u(g, P->nb, P->fl, P->d_inst, twz, P->st));
     KW_LAUNCH(launch_zfwd(g, P->nb, P->fl.p, g.ntot, P->fl.spec[0], g.nspec, twz, P->st));
@@ -593,6 +670,33 @@ int kw_plan_run(kw_plan* P, int64_t nsteps) {
   KW_HIP(hipStreamSynchronize(P->st));
   return KW_OK;
 }
+
+int kw_plan_run_profiled(kw_plan* P, int64_t nsteps, double* kernel_ms, double* kernel_bytes, int64_t* launches) {
+  if (!P || nsteps < 0 || !kernel_ms || !kernel_bytes || !launches) return fail(KW_EINVAL, "bad arguments");
+  KW_HIP(hipSetDevice(P->dev));
+  int rc;
+  if ((rc = sync_inst(P))) return rc;
+  for (int k = 0; k < KK_COUNT; ++k) { kernel_ms[k] = 0.0; kernel_bytes[k] = 0.0; launches[k] = 0; }
+  P->prof = true;
+  P->ev_used = 0;
+  P->recs.clear();
+  for (int64_t s = 0; s < nsteps && rc == KW_OK; ++s) rc = enqueue_step(P);
+  P->prof = false;
+  if (rc) return rc;
+  KW_HIP(hipStreamSynchronize(P->st));
+  for (const auto& r : P->recs) {
+    float ms = 0.f;
+    KW_HIP(hipEventElapsedTime(&ms, P->ev_pool[r.e0], P->ev_pool[r.e1]));
+    kernel_ms[r.kind] += ms;
+    kernel_bytes[r.kind] += r.bytes;
+    launches[r.kind] += 1;
+  }
+  return KW_OK;
+}
+
+int kw_kernel_kind_count(void) { return KK_COUNT; }
+
+const char* kw_kernel_kind_name(int kind) { return (kind >= 0 && kind < KK_COUNT) ? kKindNames[kind] : ""; }
 
 int kw_plan_field_size(kw_plan* P, int b, int which, int64_t* n) {
   if (!P || !n || b < 0 || b >= P->nb) return fail(KW_EINVAL, "bad plan or instance");
@@ -673,7 +777,7 @@ int kw_spectral_apply(int ndim, const int n[3], const double spacing[3], int use
     KW_LAUNCH(launch_zfwd(g, 1, fin, 0, P->fl.spec[0], 0, twz, P->st));
     if ((r = ypass(P, 0, {0}))) return r;
     static const int kinds[7] = {XK_COPY, XK_GRAD, XK_DIV, XK_FRAC1, XK_FRAC2, XK_SRCKAPPA, XK_UNSTAGGER};
-    if ((r = xpass(P, {{0, 0, kinds[kind], axis}}))) return r;
+    if ((r = xpass(P, {{0, 0, kinds[kind], axis}}, KK_XGRAD))) return r;
     if ((r = ypass(P, 1, {0}))) return r;
     KW_LAUNCH(launch_zinv(g, 1, P->fl.spec[0], 0, fin, 0, twz, 1.f, P->st));
     KW_HIP(hipStreamSynchronize(P->st));

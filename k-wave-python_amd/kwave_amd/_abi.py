@@ -23,7 +23,7 @@ EXPORTS = (
     "kw_abi_version", "kw_last_error", "kw_device_count", "kw_size_supported",
     "kw_plan_create", "kw_plan_set_instance", "kw_plan_set_field", "kw_plan_set_source",
     "kw_plan_set_sensor", "kw_plan_init", "kw_plan_run", "kw_plan_read", "kw_plan_field_size",
-    "kw_plan_destroy", "kw_spectral_apply",
+    "kw_plan_destroy", "kw_spectral_apply", "kw_plan_run_profiled", "kw_kernel_kind_count", "kw_kernel_kind_name",
 )
 
 
@@ -69,6 +69,9 @@ def lib():
         "kw_plan_field_size": ([vp, i32, i32, C.POINTER(C.c_int64)], i32),
         "kw_plan_destroy": ([vp], i32),
         "kw_spectral_apply": ([i32, C.POINTER(C.c_int), f64p, i32, C.c_double, i32, i32, C.c_double, f32p, f32p, i32], i32),
+        "kw_plan_run_profiled": ([vp, i64, f64p, f64p, i64p], i32),
+        "kw_kernel_kind_count": ([], i32),
+        "kw_kernel_kind_name": ([i32], C.c_char_p),
     }
     for name, (args, ret) in sig.items():
         fn = getattr(L, name)

@@ -111,6 +111,21 @@ class DevicePlan:
     def run(self, nsteps):
         check(_abi.lib().kw_plan_run(self._h, int(nsteps)))
 
+    def run_profiled(self, nsteps):
+        """Run nsteps with per-kernel HIP-event timing; returns {kind: (ms, bytes, launches)}."""
+        L = _abi.lib()
+        nk = L.kw_kernel_kind_count()
+        ms = np.zeros(nk)
+        by = np.zeros(nk)
+        nl = np.zeros(nk, dtype=np.int64)
+        check(L.kw_plan_run_profiled(self._h, int(nsteps), ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                     by.ctypes.data_as(C.POINTER(C.c_double)), nl.ctypes.data_as(C.POINTER(C.c_int64))))
+        return {L.kw_kernel_kind_name(k).decode(): (float(ms[k]), float(by[k]), int(nl[k])) for k in range(nk) if nl[k]}
+
+    def reset(self):
+        """Zero the state and re-apply the t = 0 initial conditions."""
+        check(_abi.lib().kw_plan_init(self._h))
+
     def read(self, b, which):
         n = C.c_int64(0)
         L = _abi.lib()
