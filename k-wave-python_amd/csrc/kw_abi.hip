@@ -97,9 +97,10 @@ Geom make_geom(int nd, const int* n) {
   g.n[0] = n[0]; g.n[1] = n[1]; g.n[2] = nd == 3 ? n[2] : 1;
   g.nz = n[nd - 1];
   g.nh = g.nz / 2 + 1;
+  g.nhp = (g.nh + 15) & ~15;
   g.ntot = (int64_t)g.n[0] * g.n[1] * g.n[2];
   g.rows = (int)(g.ntot / g.nz);
-  g.nspec = (int64_t)g.rows * g.nh;
+  g.nspec = (int64_t)g.rows * g.nhp;
   return g;
 }
 
@@ -109,11 +110,11 @@ ColArgs col_args(const Geom& g, int axis, int mode) {
   a.axis = axis;
   a.ncols = g.nh;
   if (g.nd == 2) {
-    a.estride = g.nh; a.ostride = 0; a.nouter = 1;
+    a.estride = g.nhp; a.ostride = 0; a.nouter = 1;
   } else if (axis == 1) {
-    a.estride = g.nh; a.ostride = (int64_t)g.n[1] * g.nh; a.nouter = g.n[0];
+    a.estride = g.nhp; a.ostride = (int64_t)g.n[1] * g.nhp; a.nouter = g.n[0];
   } else {
-    a.estride = (int64_t)g.n[1] * g.nh; a.ostride = g.nh; a.nouter = g.n[1];
+    a.estride = (int64_t)g.n[1] * g.nhp; a.ostride = g.nhp; a.nouter = g.n[1];
   }
   return a;
 }
@@ -225,7 +226,7 @@ int ypass(kw_plan* P, int mode, const std::vector<int>& bufs) {
   ColArgs a = col_args(P->g, 1, mode);
   a.nbuf = (int)bufs.size();
   for (size_t i = 0; i < bufs.size(); ++i) a.bufs[i] = bufs[i];
-  const double spec_bytes = 8.0 * P->nb * (double)P->g.nspec;
+  const double spec_bytes = 8.0 * P->nb * (double)P->g.rows * P->g.nh;
   KW_TLAUNCH(KK_YPASS, 2.0 * spec_bytes * bufs.size(), launch_col(P->g, P->nb, P->fl, P->ops, P->d_inst, a, P->g.n[1], P->st));
   return KW_OK;
 }
@@ -238,20 +239,20 @@ int xpass(kw_plan* P, const std::vector<XJob>& jobs, int kind) {
     a.job[i] = jobs[i];
     if (jobs[i].in != last) { ++nin; last = jobs[i].in; }
   }
-  const double spec_bytes = 8.0 * P->nb * (double)P->g.nspec;
+  const double spec_bytes = 8.0 * P->nb * (double)P->g.rows * P->g.nh;
   KW_TLAUNCH(kind, spec_bytes * (nin + jobs.size()), launch_col(P->g, P->nb, P->fl, P->ops, P->d_inst, a, P->g.n[0], P->st));
   return KW_OK;
 }
 
 // algorithmic bytes of the z-pass kernels (state arrays, medium arrays, spectra)
 double zbytes_momentum(const kw_plan* P) {
-  const double nr = (double)P->nb * P->g.ntot, ns = (double)P->nb * P->g.nspec;
+  const double nr = (double)P->nb * P->g.ntot, ns = (double)P->nb * P->g.rows * P->g.nh;
   const double dtr = P->h_inst[0].dtr0_s ? 4.0 * nr : 0.0;
   return P->g.nd * (16.0 * ns + 8.0 * nr + dtr);
 }
 double zbytes_mass(const kw_plan* P) {
   const kw::Inst& I = P->h_inst[0];
-  const double nr = (double)P->nb * P->g.ntot, ns = (double)P->nb * P->g.nspec;
+  const double nr = (double)P->nb * P->g.ntot, ns = (double)P->nb * P->g.rows * P->g.nh;
   const int D = P->g.nd;
   double b = 8.0 * ns * D + 8.0 * nr * D + (I.rho0_s ? 4.0 * nr : 0.0);
   if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP)) {
@@ -265,7 +266,7 @@ double zbytes_mass(const kw_plan* P) {
 }
 double zbytes_eos(const kw_plan* P) {
   const kw::Inst& I = P->h_inst[0];
-  const double nr = (double)P->nb * P->g.ntot, ns = (double)P->nb * P->g.nspec;
+  const double nr = (double)P->nb * P->g.ntot, ns = (double)P->nb * P->g.rows * P->g.nh;
   double b = 4.0 * nr + 4.0 * nr + 8.0 * ns;  // rho_total, p, P_z
   if (P->flags & KW_POWERLAW_ABS) b += 8.0 * ns + (I.tau_s ? 4.0 * nr : 0.0);
   if (P->flags & KW_POWERLAW_DISP) b += 8.0 * ns + (I.eta_s ? 4.0 * nr : 0.0);
@@ -310,7 +311,7 @@ int enqueue_step(kw_plan* P) {
   if ((rc = ypass(P, 0, yb))) return rc;
   if ((rc = xpass(P, jobs, KK_XDIV))) return rc;
   if ((rc = ypass(P, 1, yb))) return rc;
-  KW_TLAUNCH(KK_MASS, zbytes_mass(P), launch_mass(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st));
+  KW_TLAUNCH(KK_MASS, zbytes_mass(P), launch_mass(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st, P->chain_p));
   if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP)) {
     jobs.clear();
     yb.clear();

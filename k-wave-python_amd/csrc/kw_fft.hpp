@@ -1,27 +1,28 @@
-// LDS-resident mixed-radix Stockham FFT building blocks for gfx950.
+// Register-resident mixed-radix Stockham FFT for gfx950.
 //
-// A workgroup transforms L independent lines of length N that sit in LDS in a
-// "lines-fastest" layout:  element n of line c lives at  buf[n * LP + c]  with
-// LP = L + 1 (one float2 of padding per n-row keeps the transposing global<->LDS
-// copies free of bank conflicts).  Each stage reads R values per butterfly,
-// applies the Stockham twiddle, does an R-point DFT in registers and writes the
-// results back in place after a barrier.  The output is in natural order.
+// A workgroup transforms L independent lines of length N.  Each line is owned by
+// TPL = N / E threads; thread j0 of a line holds the E elements
+//     n = j0 + m * TPL,   m = 0 .. E-1
+// in registers (E = lcm of the radices in the plan).  With that ownership the
+// inputs of every Stockham butterfly j = j0 + q*TPL (q < E/R) of a radix-R stage
+// are already in the thread's registers (they sit at n = j + r*N/R), so a stage
+// is: twiddle + R-point DFT in registers, then one LDS exchange to move the
+// outputs to their Stockham positions before the next stage.  The last stage's
+// outputs land exactly on the owned positions, so an S-stage transform needs
+// S-1 exchanges and global loads/stores go straight from/to registers.
 //
-// Radices: 8, 4, 2, 3, 5, 7 (constexpr plan per N).  Twiddles come from a table
-// tw[m] = exp(-2 pi i m / N) built in FP64 on the host and stored as FP32.
+// Radices 8, 4, 2, 3, 5, 7.  Twiddles come from a table tw[m] = exp(-2 pi i m/N)
+// built in FP64 on the host and stored as FP32.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace kw {
-
-struct cf { float x, y; };
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 // multiply by -i (DIR=-1, forward) or +i (DIR=+1, inverse)
 template <int DIR>
@@ -42,23 +43,30 @@ struct RadixPlan {
 constexpr RadixPlan make_plan(int n) {
   RadixPlan p{n, 0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
   int m = n;
-  // powers of two: as many 8s as possible, then one 4 or 2
   while (m % 8 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 8; m /= 8; }
   if (m % 4 == 0) { p.r[p.ns++] = 4; m /= 4; }
   if (m % 2 == 0) { p.r[p.ns++] = 2; m /= 2; }
   while (m % 3 == 0) { p.r[p.ns++] = 3; m /= 3; }
   while (m % 5 == 0) { p.r[p.ns++] = 5; m /= 5; }
   while (m % 7 == 0) { p.r[p.ns++] = 7; m /= 7; }
-  if (m != 1) p.ns = -1;  // unsupported prime factor
+  if (m != 1) p.ns = -1;
   return p;
 }
-
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
 constexpr int plan_stage_radix(int n, int s) { return make_plan(n).r[s]; }
 constexpr int plan_nstages(int n) { return make_plan(n).ns; }
 constexpr int plan_ns_before(int n, int s) {
   int v = 1;
   for (int i = 0; i < s; ++i) v *= make_plan(n).r[i];
   return v;
+}
+constexpr int plan_elems(int n) {  // lcm of the radices
+  int e = 1;
+  for (int i = 0; i < make_plan(n).ns; ++i) {
+    const int r = make_plan(n).r[i];
+    e = e / gcd_c(e, r) * r;
+  }
+  return e;
 }
 
 // ---------------------------------------------------------------------------
@@ -85,28 +93,23 @@ __device__ __forceinline__ void dft4(float2* v) {
 template <int DIR>
 __device__ __forceinline__ void dft8(float2* v) {
   constexpr float h = 0.70710678118654752440f;
-  // first level: pairs (j, j+4)
   float2 a[8];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     a[j] = cadd(v[j], v[j + 4]);
     a[j + 4] = csub(v[j], v[j + 4]);
   }
-  // twiddles W8^j on the difference half (j = 1, 2, 3)
   {
     float2 t = a[5];
-    // W8^1 = (1 -/+ i)/sqrt2 for DIR -/+
     a[5] = DIR < 0 ? make_float2(h * (t.x + t.y), h * (t.y - t.x)) : make_float2(h * (t.x - t.y), h * (t.y + t.x));
     a[6] = mul_dir_i<DIR>(a[6]);
     t = a[7];
-    // W8^3 = (-1 -/+ i)/sqrt2
     a[7] = DIR < 0 ? make_float2(h * (-t.x + t.y), h * (-t.y - t.x)) : make_float2(h * (-t.x - t.y), h * (-t.y + t.x));
   }
   float2 e[4] = {a[0], a[1], a[2], a[3]};
   float2 o[4] = {a[4], a[5], a[6], a[7]};
   dft4<DIR>(e);
   dft4<DIR>(o);
-  // outputs: even indices from e, odd from o
   v[0] = e[0]; v[2] = e[1]; v[4] = e[2]; v[6] = e[3];
   v[1] = o[0]; v[3] = o[1]; v[5] = o[2]; v[7] = o[3];
 }
@@ -114,22 +117,19 @@ __device__ __forceinline__ void dft8(float2* v) {
 template <int DIR>
 __device__ __forceinline__ void dft3(float2* v) {
   constexpr float c = -0.5f;
-  constexpr float s = 0.86602540378443864676f;  // sin(2 pi/3)
+  constexpr float s = 0.86602540378443864676f;
   float2 a0 = v[0], a1 = v[1], a2 = v[2];
   float2 t1 = cadd(a1, a2);
   float2 t2 = make_float2(a0.x + c * t1.x, a0.y + c * t1.y);
   float2 d = csub(a1, a2);
-  // forward: y1 = t2 - i s d ; y2 = t2 + i s d
   float2 isd = DIR < 0 ? make_float2(s * d.y, -s * d.x) : make_float2(-s * d.y, s * d.x);
   v[0] = cadd(a0, t1);
   v[1] = cadd(t2, isd);
   v[2] = csub(t2, isd);
 }
 
-// generic odd-radix DFT from constant cos/sin tables (radix 5 and 7)
 template <int R, int DIR>
 __device__ __forceinline__ void dft_odd(float2* v) {
-  // cos(2 pi m / R), sin(2 pi m / R) for m = 0..R-1
   constexpr float c5[5] = {1.0f, 0.30901699437494742f, -0.80901699437494742f, -0.80901699437494742f, 0.30901699437494742f};
   constexpr float s5[5] = {0.0f, 0.95105651629515357f, 0.58778525229247313f, -0.58778525229247313f, -0.95105651629515357f};
   constexpr float c7[7] = {1.0f, 0.62348980185873353f, -0.22252093395631440f, -0.90096886790241913f,
@@ -164,69 +164,94 @@ __device__ __forceinline__ void dft(float2* v) {
 }
 
 // ---------------------------------------------------------------------------
-// one Stockham stage over L lines of length N in LDS (in place)
+// LDS index maps used for the exchanges
 // ---------------------------------------------------------------------------
-template <int N, int L, int NT, int R, int NS, int DIR>
-__device__ __forceinline__ void stockham_stage(float2* buf, const float2* __restrict__ tw, int tid) {
-  constexpr int LP = L + 1;
-  constexpr int NB = N / R;            // butterflies per line
-  constexpr int TOT = NB * L;          // butterflies per workgroup
-  constexpr int PER = (TOT + NT - 1) / NT;
-  constexpr int TWS = N / (NS * R);    // twiddle stride in the length-N table
-  float2 v[PER][R];
+// lines-fastest (column passes: consecutive threads = consecutive columns)
+template <int L>
+struct ColIdx {
+  __device__ __forceinline__ int operator()(int n, int c) const { return n * (L + 1) + c; }
+  static constexpr int size(int N) { return N * (L + 1); }
+};
+// line-major with one pad slot every 8 elements (row passes: consecutive
+// threads = consecutive positions of one line)
+template <int N>
+struct RowIdx {
+  static constexpr int LS = N + N / 8 + 1;
+  __device__ __forceinline__ int operator()(int n, int c) const { return c * LS + n + (n >> 3); }
+  static constexpr int size(int L) { return L * LS; }
+};
+
+// EM multiplies the per-thread element count (more loads in flight per thread,
+// fewer threads per line); N must stay divisible by E.
+template <int N, int EM = 1>
+struct FftShape {
+  static constexpr int E = plan_elems(N) * EM;
+  static constexpr int TPL = N / E;
+  static constexpr int S = plan_nstages(N);
+  static_assert(N % E == 0, "elements per thread must divide the line length");
+};
+
+// One stage on the owned elements v (see header); Idx maps (n, line) -> LDS slot.
+template <int N, int EM, int S, int DIR, class Idx>
+__device__ __forceinline__ void reg_stage(float2 (&v)[(FftShape<N, EM>::E)], float2* lds, const float2* __restrict__ tw, int c, int j0,
+                                          const Idx& idx) {
+  constexpr int E = FftShape<N, EM>::E;
+  constexpr int TPL = FftShape<N, EM>::TPL;
+  constexpr int R = plan_stage_radix(N, S);
+  constexpr int NS = plan_ns_before(N, S);
+  constexpr int Q = E / R;
+  constexpr int TWS = N / (NS * R);
+  constexpr bool LAST = (S == plan_nstages(N) - 1);
 #pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int b = tid + i * NT;
-    if (TOT % NT == 0 || b < TOT) {
-      const int c = b % L;
-      const int j = b / L;
+  for (int q = 0; q < Q; ++q) {
+    const int j = j0 + q * TPL;
+    float2 x[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * NB) * LP + c];
-      if (NS > 1) {
-        const int k = j % NS;
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          float2 w = tw[(r * k * TWS) % N];
-          if (DIR > 0) w.y = -w.y;
-          v[i][r] = cmul(v[i][r], w);
-        }
-      }
-      dft<R, DIR>(v[i]);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int b = tid + i * NT;
-    if (TOT % NT == 0 || b < TOT) {
-      const int c = b % L;
-      const int j = b / L;
+    for (int r = 0; r < R; ++r) x[r] = v[q + r * Q];
+    if constexpr (NS > 1) {
       const int k = j % NS;
-      const int o = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[(o + r * NS) * LP + c] = v[i][r];
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[r * k * TWS];
+        if (DIR > 0) w.y = -w.y;
+        x[r] = cmul(x[r], w);
+      }
+    }
+    dft<R, DIR>(x);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q + r * Q] = x[r];
+    } else {
+      const int o = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[idx(o + r * NS, c)] = x[r];
     }
   }
-  __syncthreads();
-}
-
-template <int N, int L, int NT, int DIR, int S>
-__device__ __forceinline__ void fft_stages(float2* buf, const float2* __restrict__ tw, int tid) {
-  if constexpr (S < plan_nstages(N)) {
-    constexpr int R = plan_stage_radix(N, S);
-    constexpr int NS = plan_ns_before(N, S);
-    stockham_stage<N, L, NT, R, NS, DIR>(buf, tw, tid);
-    fft_stages<N, L, NT, DIR, S + 1>(buf, tw, tid);
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = lds[idx(j0 + m * TPL, c)];
+    __syncthreads();
   }
 }
 
-// Full in-LDS FFT of L lines (caller must __syncthreads() before, data in
-// natural order after; ends with a barrier).  DIR = -1 forward, +1 inverse
-// (unnormalised).
-template <int N, int L, int NT, int DIR>
-__device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw, int tid) {
+template <int N, int EM, int S, int DIR, class Idx>
+__device__ __forceinline__ void reg_stages(float2 (&v)[(FftShape<N, EM>::E)], float2* lds, const float2* __restrict__ tw, int c, int j0,
+                                           const Idx& idx) {
+  if constexpr (S < plan_nstages(N)) {
+    reg_stage<N, EM, S, DIR>(v, lds, tw, c, j0, idx);
+    reg_stages<N, EM, S + 1, DIR>(v, lds, tw, c, j0, idx);
+  }
+}
+
+// Transform the owned elements of line c in place (natural order in, natural
+// order out at the same owned positions).  Every thread of the workgroup must
+// call it (it contains barriers); the LDS buffer must not be in use.
+template <int N, int DIR, class Idx, int EM = 1>
+__device__ __forceinline__ void fft_reg(float2 (&v)[(FftShape<N, EM>::E)], float2* lds, const float2* __restrict__ tw, int c, int j0,
+                                        const Idx& idx) {
   static_assert(plan_nstages(N) > 0, "unsupported FFT length");
-  fft_stages<N, L, NT, DIR, 0>(buf, tw, tid);
+  reg_stages<N, EM, 0, DIR>(v, lds, tw, c, j0, idx);
 }
 
 }  // namespace kw

@@ -1,0 +1,634 @@
+// Time-loop kernels of the k-space pseudospectral solver (gfx950).
+//
+// Every kernel works on a *batch* of same-shape instances (blockIdx.y is the
+// instance) and reads the instance's step counter from device memory, so that
+// one captured sequence serves every step and finished instances drop out.
+//
+// z-pass kernels own the contiguous axis: a workgroup takes L pairs of grid
+// rows, runs the real<->half-complex transform of both rows of a pair as one
+// complex FFT (row a in the real part, row b in the imaginary part), and fuses
+// the pointwise physics of the step between the inverse and forward transforms.
+// Column kernels own the strided axes; the outermost-axis pass applies the
+// spectral multipliers (regenerated in registers from 1-D tables) between its
+// forward and inverse transforms.
+#pragma once
+#include "kw_fft.hpp"
+#include "kw_launch.hpp"
+#include "kw_types.hpp"
+
+namespace kw {
+
+// ---------------------------------------------------------------------------
+// z-pass configuration
+// ---------------------------------------------------------------------------
+constexpr int pow2floor(int v) {
+  int p = 1;
+  while (p * 2 <= v) p *= 2;
+  return p;
+}
+constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <int N>
+struct ZCfg {
+  using F = FftShape<N>;
+  static constexpr int E = F::E;
+  static constexpr int TPL = F::TPL;
+  static constexpr int L0 = clampi(pow2floor(2048 / N), 1, 64);
+  static constexpr int L = (L0 * TPL < 64) ? clampi(64 / TPL, L0, 64) : L0;  // row pairs per workgroup
+  static constexpr int NT = TPL * L;
+  static constexpr int NH = N / 2 + 1;
+  using Idx = RowIdx<N>;
+  static constexpr int LDS = Idx::size(L);
+};
+
+__device__ __forceinline__ float ld(const float* p, int64_t i, int s) { return p[i * s]; }
+
+// pml value of axis `ax` at grid row r, contiguous position n
+__device__ __forceinline__ float pml_at(const float* const* tab, const Geom& g, int ax, int r, int n) {
+  if (ax == g.nd - 1) return tab[ax][n];
+  if (ax == 0) return tab[0][g.nd == 2 ? r : r / g.n[1]];
+  return tab[1][r % g.n[1]];
+}
+
+// Load the half spectra of the L row pairs of this workgroup into LDS as the
+// full spectrum of z = a + i b (Hermitian extension of each half spectrum).
+template <int N>
+__device__ __forceinline__ void load_pair_spec(float2* lds, const float2* __restrict__ spec, int row0, int rows, int pitch, int tid) {
+  using C = ZCfg<N>;
+  const typename C::Idx idx{};
+  for (int e = tid; e < C::L * C::NH; e += C::NT) {
+    const int p = e / C::NH, k = e % C::NH;
+    const int ra = row0 + 2 * p, rb = ra + 1;
+    float2 A = ra < rows ? spec[(int64_t)ra * pitch + k] : make_float2(0.f, 0.f);
+    float2 B = rb < rows ? spec[(int64_t)rb * pitch + k] : make_float2(0.f, 0.f);
+    if (k == 0 || (N % 2 == 0 && k == N / 2)) {
+      lds[idx(k, p)] = make_float2(A.x, B.x);
+    } else {
+      lds[idx(k, p)] = make_float2(A.x - B.y, A.y + B.x);
+      lds[idx(N - k, p)] = make_float2(A.x + B.y, B.x - A.y);
+    }
+  }
+}
+
+// After a forward FFT of z = a + i b (in LDS), split into the two half spectra.
+template <int N>
+__device__ __forceinline__ void store_pair_spec(const float2* lds, float2* __restrict__ spec, int row0, int rows, int pitch, int tid) {
+  using C = ZCfg<N>;
+  const typename C::Idx idx{};
+  for (int e = tid; e < C::L * C::NH; e += C::NT) {
+    const int p = e / C::NH, k = e % C::NH;
+    const int ra = row0 + 2 * p, rb = ra + 1;
+    const float2 Z = lds[idx(k, p)];
+    const float2 Zc = lds[idx((N - k) % N, p)];
+    if (ra < rows) spec[(int64_t)ra * pitch + k] = make_float2(0.5f * (Z.x + Zc.x), 0.5f * (Z.y - Zc.y));
+    if (rb < rows) spec[(int64_t)rb * pitch + k] = make_float2(0.5f * (Z.y + Zc.y), -0.5f * (Z.x - Zc.x));
+  }
+}
+
+// half spectra of a row pair -> real values of both rows at the owned positions
+template <int N>
+__device__ __forceinline__ void inv_to_regs(float2* lds, const float2* __restrict__ spec, const float2* __restrict__ tw,
+                                            int row0, int rows, int pitch, int tid, float2 (&out)[ZCfg<N>::E]) {
+  using C = ZCfg<N>;
+  const typename C::Idx idx{};
+  const int j0 = tid % C::TPL, c = tid / C::TPL;
+  __syncthreads();
+  load_pair_spec<N>(lds, spec, row0, rows, pitch, tid);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) out[m] = lds[idx(j0 + m * C::TPL, c)];
+  __syncthreads();
+  fft_reg<N, +1>(out, lds, tw, c, j0, idx);
+}
+
+// real values of a row pair at the owned positions -> half spectra of both rows
+template <int N>
+__device__ __forceinline__ void fwd_from_regs(float2* lds, float2 (&v)[ZCfg<N>::E], float2* __restrict__ spec,
+                                              const float2* __restrict__ tw, int row0, int rows, int pitch, int tid) {
+  using C = ZCfg<N>;
+  const typename C::Idx idx{};
+  const int j0 = tid % C::TPL, c = tid / C::TPL;
+  fft_reg<N, -1>(v, lds, tw, c, j0, idx);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) lds[idx(j0 + m * C::TPL, c)] = v[m];
+  __syncthreads();
+  store_pair_spec<N>(lds, spec, row0, rows, pitch, tid);
+}
+
+// Source value added/imposed at (row r, position n) for a CSR source term.
+// Returns true when (r, n) is a source point; v receives scale * signal(t).
+__device__ __forceinline__ bool csr_lookup(const Src& s, int r, int n, int t, float& v) {
+  int lo = s.row_ptr[r], hi = s.row_ptr[r + 1];
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const int c = s.col[mid];
+    if (c == n) {
+      v = s.scale[mid] * s.sig[(int64_t)s.sigrow[mid] * s.len + t];
+      return true;
+    }
+    if (c < n) lo = mid + 1; else hi = mid;
+  }
+  return false;
+}
+
+// Apply a source term to field value f at flat index idx (row r, position n),
+// axis factor fac. chain holds the inverse-transformed filtered source (mode CHAIN).
+__device__ __forceinline__ float apply_src(const Src& s, float f, int64_t idx, int r, int n, int t, float fac, float chain) {
+  if (s.mode == SRC_NONE || t >= s.len) return f;
+  if (s.mode == SRC_PATTERN) return f + fac * s.sig[t] * s.pattern[idx];
+  if (s.mode == SRC_CHAIN) return f + fac * chain;
+  float v;
+  if (!csr_lookup(s, r, n, t, v)) return f;
+  return s.mode == SRC_DIRICHLET ? fac * v : f + fac * v;
+}
+
+#define KW_ZPASS_PROLOGUE                                              \
+  using C = ZCfg<N>;                                                   \
+  __shared__ float2 lds[C::LDS];                                       \
+  __shared__ float2 twl[N];                                            \
+  const int tid = threadIdx.x;                                         \
+  for (int i_ = tid; i_ < N; i_ += C::NT) twl[i_] = tw_g[i_];          \
+  __syncthreads();                                                     \
+  const float2* tw = twl;                                              \
+  const int b = blockIdx.y;                                            \
+  const int row0 = blockIdx.x * 2 * C::L;                              \
+  const int rows = g.rows;                                             \
+  const int j0 = tid % C::TPL, pc = tid / C::TPL;                      \
+  const int ra = row0 + 2 * pc;                                        \
+  (void)j0; (void)ra;
+
+// ---------------------------------------------------------------------------
+// generic real <-> half-spectrum transforms along the contiguous axis
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(ZCfg<N>::NT) void k_zfwd(Geom g, const float* __restrict__ f, int64_t fstride, float2* __restrict__ spec,
+                                                      int64_t sstride, const float2* __restrict__ tw_g) {
+  KW_ZPASS_PROLOGUE
+  const float* fb = f + b * fstride;
+  float2 v[C::E];
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const int n = j0 + m * C::TPL;
+    v[m] = make_float2(ra < rows ? fb[(int64_t)ra * N + n] : 0.f, ra + 1 < rows ? fb[(int64_t)(ra + 1) * N + n] : 0.f);
+  }
+  fwd_from_regs<N>(lds, v, spec + b * sstride, tw, row0, rows, g.nhp, tid);
+}
+
+template <int N>
+__global__ __launch_bounds__(ZCfg<N>::NT) void k_zinv(Geom g, const float2* __restrict__ spec, int64_t sstride, float* __restrict__ f,
+                                                      int64_t fstride, const float2* __restrict__ tw_g, float scale) {
+  KW_ZPASS_PROLOGUE
+  float* fb = f + b * fstride;
+  float2 v[C::E];
+  inv_to_regs<N>(lds, spec + b * sstride, tw, row0, rows, g.nhp, tid, v);
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const int n = j0 + m * C::TPL;
+    if (ra < rows) fb[(int64_t)ra * N + n] = scale * v[m].x;
+    if (ra + 1 < rows) fb[(int64_t)(ra + 1) * N + n] = scale * v[m].y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// momentum: u_i = pml_sg (pml_sg u_i - dt/rho0_sg * grad_i p) (+ source), then
+// forward transform of u_i along z (for the divergence).  blockIdx.z = axis.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(ZCfg<N>::NT) void k_momentum(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g) {
+  KW_ZPASS_PROLOGUE
+  const int ax = blockIdx.z;
+  const Inst& I = inst[b];
+  const int t = fl.tdev[b];
+  if (t >= I.nt) return;
+  float2* sp = fl.spec[ax] + b * g.nspec;
+  float* u = fl.u[ax] + b * g.ntot;
+  const Src& src = I.srcu[ax];
+  // operands of the update, fetched before the transform so their latency overlaps it
+  float2 uo[C::E], dr[C::E];
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const int n = j0 + m * C::TPL;
+    const int64_t i0 = (int64_t)ra * N + n;
+    const bool h0 = ra < rows, h1 = ra + 1 < rows;
+    uo[m] = make_float2(h0 ? u[i0] : 0.f, h1 ? u[i0 + N] : 0.f);
+    dr[m] = make_float2(h0 ? ld(I.dtr0[ax], i0, I.dtr0_s) : 0.f, h1 ? ld(I.dtr0[ax], i0 + N, I.dtr0_s) : 0.f);
+  }
+  float2 gv[C::E];
+  inv_to_regs<N>(lds, sp, tw, row0, rows, g.nhp, tid, gv);
+  float2 ch[C::E];
+  const bool chain = src.mode == SRC_CHAIN && t < src.len;
+  if (chain) inv_to_regs<N>(lds, fl.spec[src.chain_buf] + b * g.nspec, tw, row0, rows, g.nhp, tid, ch);
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const int n = j0 + m * C::TPL;
+    float res[2] = {0.f, 0.f};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = ra + h;
+      if (r >= rows) continue;
+      const int64_t idx = (int64_t)r * N + n;
+      const float s = pml_at(I.pmlsg, g, ax, r, n);
+      const float gval = h ? gv[m].y : gv[m].x;
+      float uv = s * (s * (h ? uo[m].y : uo[m].x) - (h ? dr[m].y : dr[m].x) * gval);
+      uv = apply_src(src, uv, idx, r, n, t, src.fac[ax], chain ? (h ? ch[m].y : ch[m].x) : 0.f);
+      u[idx] = uv;
+      res[h] = uv;
+    }
+    gv[m] = make_float2(res[0], res[1]);
+  }
+  fwd_from_regs<N>(lds, gv, sp, tw, row0, rows, g.nhp, tid);
+}
+
+// ---------------------------------------------------------------------------
+// mass conservation (+ equation of state when no fractional Laplacian is needed)
+// ---------------------------------------------------------------------------
+template <int N, int ND, bool CHAIN>
+__global__ __launch_bounds__(ZCfg<N>::NT) void k_mass(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g, int flags) {
+  KW_ZPASS_PROLOGUE
+  const Inst& I = inst[b];
+  const int t = fl.tdev[b];
+  if (t >= I.nt) return;
+  const int64_t off = b * g.ntot;
+  const Src& src = I.srcp;
+  const bool chain = CHAIN && src.mode == SRC_CHAIN && t < src.len;
+  const bool powerlaw = flags & (FLAG_POWERLAW_ABS | FLAG_POWERLAW_DISP);
+  // per owned element and row: rho0, the nonlinear factor (from the old split
+  // densities), and the running sums of div u and of the new densities
+  float2 r0v[C::E], nlv[C::E], divu[C::E], rt[C::E];
+  float2 ch[CHAIN ? C::E : 1];
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const int n = j0 + m * C::TPL;
+    float r0h[2] = {1.f, 1.f}, nlh[2] = {1.f, 1.f};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = ra + h;
+      if (r >= rows) continue;
+      const int64_t idx = (int64_t)r * N + n;
+      r0h[h] = ld(I.rho0, idx, I.rho0_s);
+      if (flags & FLAG_NONLINEAR) {
+        float rs = 0.f;
+#pragma unroll
+        for (int i = 0; i < ND; ++i) rs += fl.rho[i][off + idx];
+        nlh[h] = (2.f * rs + r0h[h]) / r0h[h];
+      }
+    }
+    r0v[m] = make_float2(r0h[0], r0h[1]);
+    nlv[m] = make_float2(nlh[0], nlh[1]);
+    divu[m] = make_float2(0.f, 0.f);
+    rt[m] = make_float2(0.f, 0.f);
+  }
+  if constexpr (CHAIN) {
+    if (chain) inv_to_regs<N>(lds, fl.spec[src.chain_buf] + b * g.nspec, tw, row0, rows, g.nhp, tid, ch);
+  }
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    float2 d[C::E];
+    inv_to_regs<N>(lds, fl.spec[i] + b * g.nspec, tw, row0, rows, g.nhp, tid, d);
+#pragma unroll
+    for (int m = 0; m < C::E; ++m) {
+      const int n = j0 + m * C::TPL;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = ra + h;
+        if (r >= rows) continue;
+        const int64_t idx = (int64_t)r * N + n;
+        const float dv = h ? d[m].y : d[m].x;
+        const float s = pml_at(I.pml, g, i, r, n);
+        const float r0 = h ? r0v[m].y : r0v[m].x;
+        const float nl = h ? nlv[m].y : nlv[m].x;
+        float rv = s * (s * fl.rho[i][off + idx] - I.dt * r0 * dv * nl);
+        float chv = 0.f;
+        if constexpr (CHAIN) chv = chain ? (h ? ch[m].y : ch[m].x) : 0.f;
+        rv = apply_src(src, rv, idx, r, n, t, src.fac[i], chv);
+        fl.rho[i][off + idx] = rv;
+        if (h) { divu[m].y += dv; rt[m].y += rv; } else { divu[m].x += dv; rt[m].x += rv; }
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const int n = j0 + m * C::TPL;
+    float a_out[2] = {0.f, 0.f};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = ra + h;
+      if (r >= rows) continue;
+      const int64_t idx = (int64_t)r * N + n;
+      const float r0 = h ? r0v[m].y : r0v[m].x;
+      const float du = h ? divu[m].y : divu[m].x;
+      const float rtv = h ? rt[m].y : rt[m].x;
+      if (powerlaw) {
+        fl.rhot[off + idx] = rtv;
+        a_out[h] = r0 * du;
+      } else {
+        float q = rtv;
+        if (flags & FLAG_STOKES) q += ld(I.tau, idx, I.tau_s) * r0 * du;
+        if (flags & FLAG_NONLINEAR) q += ld(I.bona, idx, I.bona_s) * rtv * rtv / (2.f * r0);
+        const float pv = ld(I.c0sq, idx, I.c0sq_s) * q;
+        fl.p[off + idx] = pv;
+        a_out[h] = pv;
+      }
+    }
+    divu[m] = make_float2(a_out[0], a_out[1]);
+  }
+  if (!powerlaw) {
+    fwd_from_regs<N>(lds, divu, fl.spec[0] + b * g.nspec, tw, row0, rows, g.nhp, tid);
+  } else {
+    if (flags & FLAG_POWERLAW_ABS) fwd_from_regs<N>(lds, divu, fl.spec[0] + b * g.nspec, tw, row0, rows, g.nhp, tid);
+    if (flags & FLAG_POWERLAW_DISP) fwd_from_regs<N>(lds, rt, fl.spec[1] + b * g.nspec, tw, row0, rows, g.nhp, tid);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// equation of state with power-law absorption / dispersion
+// p = c0^2 (rho + tau F^-1[|k|^{y-2} F[rho0 div u]] - eta F^-1[|k|^{y-1} F[rho]] + B/A rho^2 / (2 rho0))
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(ZCfg<N>::NT) void k_eos(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g, int flags) {
+  KW_ZPASS_PROLOGUE
+  const Inst& I = inst[b];
+  const int t = fl.tdev[b];
+  if (t >= I.nt) return;
+  const int64_t off = b * g.ntot;
+  float2 rtv[C::E], c2v[C::E];
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const int64_t i0 = (int64_t)ra * N + j0 + m * C::TPL;
+    const bool h0 = ra < rows, h1 = ra + 1 < rows;
+    rtv[m] = make_float2(h0 ? fl.rhot[off + i0] : 0.f, h1 ? fl.rhot[off + i0 + N] : 0.f);
+    c2v[m] = make_float2(h0 ? ld(I.c0sq, i0, I.c0sq_s) : 0.f, h1 ? ld(I.c0sq, i0 + N, I.c0sq_s) : 0.f);
+  }
+  float2 av[C::E], bv[C::E];
+  if (flags & FLAG_POWERLAW_ABS) inv_to_regs<N>(lds, fl.spec[0] + b * g.nspec, tw, row0, rows, g.nhp, tid, av);
+  if (flags & FLAG_POWERLAW_DISP) inv_to_regs<N>(lds, fl.spec[1] + b * g.nspec, tw, row0, rows, g.nhp, tid, bv);
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const int n = j0 + m * C::TPL;
+    float out[2] = {0.f, 0.f};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = ra + h;
+      if (r >= rows) continue;
+      const int64_t idx = (int64_t)r * N + n;
+      const float rt = h ? rtv[m].y : rtv[m].x;
+      float q = rt;
+      if (flags & FLAG_POWERLAW_ABS) q += ld(I.tau, idx, I.tau_s) * (h ? av[m].y : av[m].x);
+      if (flags & FLAG_POWERLAW_DISP) q -= ld(I.eta, idx, I.eta_s) * (h ? bv[m].y : bv[m].x);
+      if (flags & FLAG_NONLINEAR) q += ld(I.bona, idx, I.bona_s) * rt * rt / (2.f * ld(I.rho0, idx, I.rho0_s));
+      const float pv = (h ? c2v[m].y : c2v[m].x) * q;
+      fl.p[off + idx] = pv;
+      out[h] = pv;
+    }
+    av[m] = make_float2(out[0], out[1]);
+  }
+  fwd_from_regs<N>(lds, av, fl.spec[0] + b * g.nspec, tw, row0, rows, g.nhp, tid);
+}
+
+// ---------------------------------------------------------------------------
+// t = 0 initial-value override: u_i = dt/(2 rho0_sg_i) grad_i p0, rho_j = p0 / (c0^2 D)
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(ZCfg<N>::NT) void k_init_u(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g) {
+  KW_ZPASS_PROLOGUE
+  const int ax = blockIdx.z;
+  const Inst& I = inst[b];
+  const int64_t off = b * g.ntot;
+  float2 gv[C::E];
+  inv_to_regs<N>(lds, fl.spec[ax] + b * g.nspec, tw, row0, rows, g.nhp, tid, gv);
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const int n = j0 + m * C::TPL;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = ra + h;
+      if (r >= rows) continue;
+      const int64_t idx = (int64_t)r * N + n;
+      fl.u[ax][off + idx] = 0.5f * ld(I.dtr0[ax], idx, I.dtr0_s) * (h ? gv[m].y : gv[m].x);
+      if (ax == 0) {
+        const float rv = fl.p[off + idx] / (ld(I.c0sq, idx, I.c0sq_s) * (float)g.nd);
+        for (int j = 0; j < g.nd; ++j) fl.rho[j][off + idx] = rv;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// build the z-spectrum of a multi-row additive source at step t (mode CHAIN)
+// which = -1: pressure source, 0..2: velocity component
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(ZCfg<N>::NT) void k_src_build(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g, int which) {
+  KW_ZPASS_PROLOGUE
+  const Inst& I = inst[b];
+  const int t = fl.tdev[b];
+  const Src& s = which < 0 ? I.srcp : I.srcu[which];
+  if (t >= I.nt || s.mode != SRC_CHAIN || t >= s.len) return;
+  float2* sp = fl.spec[s.chain_buf] + b * g.nspec;
+  const int rend = min(row0 + 2 * C::L, rows);
+  const int e0 = s.row_ptr[row0], e1 = s.row_ptr[rend];
+  if (e0 == e1) {
+    for (int e = tid; e < C::L * C::NH; e += C::NT) {
+      const int p = e / C::NH, k = e % C::NH;
+      const int r0 = row0 + 2 * p;
+      if (r0 < rows) sp[(int64_t)r0 * g.nhp + k] = make_float2(0.f, 0.f);
+      if (r0 + 1 < rows) sp[(int64_t)(r0 + 1) * g.nhp + k] = make_float2(0.f, 0.f);
+    }
+    return;
+  }
+  const typename C::Idx idx{};
+  for (int e = tid; e < N * C::L; e += C::NT) lds[idx(e % N, e / N)] = make_float2(0.f, 0.f);
+  __syncthreads();
+  float* fb = reinterpret_cast<float*>(lds);
+  for (int e = e0 + tid; e < e1; e += C::NT) {
+    int r = row0;
+    while (s.row_ptr[r + 1] <= e) ++r;
+    const int p = (r - row0) >> 1, h = (r - row0) & 1;
+    fb[2 * idx(s.col[e], p) + h] = s.scale[e] * s.sig[(int64_t)s.sigrow[e] * s.len + t];
+  }
+  __syncthreads();
+  float2 v[C::E];
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) v[m] = lds[idx(j0 + m * C::TPL, pc)];
+  __syncthreads();
+  fwd_from_regs<N>(lds, v, sp, tw, row0, rows, g.nhp, tid);
+}
+
+// ---------------------------------------------------------------------------
+// strided column passes
+// ---------------------------------------------------------------------------
+// sin(x)/x and cos(x) by their Taylor series through x^14 (truncation error
+// below 2e-10 for |x| <= 2, which covers c_ref dt |k| / 2 at any stable CFL);
+// larger arguments fall back to the library functions.
+__device__ __forceinline__ float sinc_k(float x) {
+  const float x2 = x * x;
+  if (x2 > 4.f) return sinf(x) / x;
+  float p = -1.f / 1307674368000.f;                 // -1/15!
+  p = fmaf(p, x2, 1.f / 6227020800.f);              //  1/13!
+  p = fmaf(p, x2, -1.f / 39916800.f);               // -1/11!
+  p = fmaf(p, x2, 1.f / 362880.f);                  //  1/9!
+  p = fmaf(p, x2, -1.f / 5040.f);                   // -1/7!
+  p = fmaf(p, x2, 1.f / 120.f);                     //  1/5!
+  p = fmaf(p, x2, -1.f / 6.f);                      // -1/3!
+  return fmaf(p, x2, 1.f);
+}
+__device__ __forceinline__ float cos_k(float x) {
+  const float x2 = x * x;
+  if (x2 > 4.f) return cosf(x);
+  float p = -1.f / 87178291200.f;                   // -1/14!
+  p = fmaf(p, x2, 1.f / 479001600.f);               //  1/12!
+  p = fmaf(p, x2, -1.f / 3628800.f);                // -1/10!
+  p = fmaf(p, x2, 1.f / 40320.f);                   //  1/8!
+  p = fmaf(p, x2, -1.f / 720.f);                    // -1/6!
+  p = fmaf(p, x2, 1.f / 24.f);                      //  1/4!
+  p = fmaf(p, x2, -0.5f);                           // -1/2!
+  return fmaf(p, x2, 1.f);
+}
+// |k|^a for |k| > 0 (the fractional Laplacian is 0 at |k| = 0)
+__device__ __forceinline__ float kpow(float km, float a) { return km == 0.f ? 0.f : exp2f(a * log2f(km)); }
+
+// Consecutive workgroup ids go to different XCDs (round robin over 8); give each
+// XCD a contiguous range of tiles so neighbouring tiles share its L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  const int nfull = nblk & ~7;
+  if (bid >= nfull) return bid;
+  return (bid & 7) * (nfull >> 3) + (bid >> 3);
+}
+
+template <int N, int L, int EM = 1>
+struct ColCfg {
+  using F = FftShape<N, EM>;
+  static constexpr int E = F::E;
+  static constexpr int TPL = F::TPL;
+  static constexpr int NT = TPL * L;
+  using Idx = ColIdx<L>;
+  static constexpr int LDS = Idx::size(N);
+};
+
+template <int N>
+constexpr int col_lbig() { return clampi(pow2floor(8192 / N), 1, 64); }
+template <int N>
+constexpr int col_lsmall() { return col_lbig<N>() / 4 < 1 ? 1 : col_lbig<N>() / 4; }
+
+template <int N, int L, bool OPS, int EM = 1>
+__global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) void k_col(Geom g, Fields fl, Ops ops, const Inst* __restrict__ inst, ColArgs a) {
+  using C = ColCfg<N, L, EM>;
+  __shared__ float2 lds[C::LDS];
+  __shared__ float2 twl[N];
+  const typename C::Idx idx{};
+  const int tid = threadIdx.x;
+  const int b = blockIdx.y;
+  if (fl.tdev && fl.tdev[b] >= inst[b].nt) return;
+  for (int i_ = tid; i_ < N; i_ += C::NT) twl[i_] = ops.tw[a.axis][i_];
+  __syncthreads();
+  const int ntile = (a.ncols + L - 1) / L;
+  const int lid = xcd_remap(blockIdx.x, gridDim.x);
+  const int o = lid / ntile, tile = lid % ntile;
+  const int c = tid % L, j0 = tid / L;
+  const int col = tile * L + c;
+  const bool valid = col < a.ncols;
+  const int64_t base = (int64_t)b * g.nspec + (int64_t)o * a.ostride + col;
+  const float2* tw = twl;
+
+  float2 v[C::E];
+  if constexpr (!OPS) {
+    // software pipeline over the buffers: the loads of buffer q+1 are in flight
+    // while buffer q is transformed
+    float2 nx[C::E];
+    if (valid) {
+      const float2* bp = fl.spec[a.bufs[0]];
+#pragma unroll
+      for (int m = 0; m < C::E; ++m) nx[m] = bp[base + (int64_t)(j0 + m * C::TPL) * a.estride];
+    }
+    for (int q = 0; q < a.nbuf; ++q) {
+      float2* bp = fl.spec[a.bufs[q]];
+#pragma unroll
+      for (int m = 0; m < C::E; ++m) v[m] = valid ? nx[m] : make_float2(0.f, 0.f);
+      if (q + 1 < a.nbuf && valid) {
+        const float2* bn = fl.spec[a.bufs[q + 1]];
+#pragma unroll
+        for (int m = 0; m < C::E; ++m) nx[m] = bn[base + (int64_t)(j0 + m * C::TPL) * a.estride];
+      }
+      if (a.mode == 0) fft_reg<N, -1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
+      else fft_reg<N, +1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
+      if (valid) {
+#pragma unroll
+        for (int m = 0; m < C::E; ++m) bp[base + (int64_t)(j0 + m * C::TPL) * a.estride] = v[m];
+      }
+    }
+  } else {
+  // forward, multiply, inverse (outermost axis = axis 0)
+  const float kap = inst[b].kap;
+  int iy, iz;
+  if (g.nd == 2) { iy = valid ? col : 0; iz = 0; }
+  else { iy = o; iz = valid ? col : 0; }
+  float kyz2 = 0.f;
+  {
+    const float ky = ops.k[1][iy];
+    kyz2 = ky * ky;
+    if (g.nd == 3) { const float kz = ops.k[2][iz]; kyz2 += kz * kz; }
+  }
+  float2 st[C::E];
+  int cur_in = -1;
+  for (int j = 0; j < a.njob; ++j) {
+    const XJob jb = a.job[j];
+    if (jb.in != cur_in) {
+      cur_in = jb.in;
+      const float2* src = fl.spec[jb.in];
+#pragma unroll
+      for (int m = 0; m < C::E; ++m) st[m] = valid ? src[base + (int64_t)(j0 + m * C::TPL) * a.estride] : make_float2(0.f, 0.f);
+      fft_reg<N, -1, typename C::Idx, EM>(st, lds, tw, c, j0, idx);
+    }
+    const int ixa = jb.axis == 0 ? -1 : (jb.axis == 1 ? iy : iz);
+#pragma unroll
+    for (int m = 0; m < C::E; ++m) {
+      const int n = j0 + m * C::TPL;
+      const int ia = ixa < 0 ? n : ixa;
+      const float kx = ops.k[0][n];
+      const float km = sqrtf(fmaf(kx, kx, kyz2));
+      float2 op;
+      switch (jb.kind) {
+        case XK_GRAD: op = cscale(ops.grad[jb.axis][ia], kap > 0.f ? sinc_k(kap * km) : 1.f); break;
+        case XK_DIV: op = cscale(ops.div[jb.axis][ia], kap > 0.f ? sinc_k(kap * km) : 1.f); break;
+        case XK_FRAC1: op = make_float2(kpow(km, ops.frac1), 0.f); break;
+        case XK_FRAC2: op = make_float2(kpow(km, ops.frac2), 0.f); break;
+        case XK_SRCKAPPA: op = make_float2(kap > 0.f ? cos_k(kap * km) : 1.f, 0.f); break;
+        case XK_UNSTAGGER: op = ops.unst[jb.axis][ia]; break;
+        default: op = make_float2(1.f, 0.f); break;
+      }
+      v[m] = cscale(cmul(st[m], op), ops.inv_ntot);
+    }
+    fft_reg<N, +1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
+    if (valid) {
+      float2* dst = fl.spec[jb.out];
+#pragma unroll
+      for (int m = 0; m < C::E; ++m) dst[base + (int64_t)(j0 + m * C::TPL) * a.estride] = v[m];
+    }
+  }
+  }
+}
+
+#define KW_FOR_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(24) X(48) X(96) X(192) X(384) X(768) X(40) X(80) X(160) X(320) X(640) X(56) X(112) X(224) X(448)
+
+inline bool size_supported_impl(int n) {
+  switch (n) {
+#define KW_CASE(S) case S: return true;
+    KW_FOR_SIZES(KW_CASE)
+#undef KW_CASE
+    default: return false;
+  }
+}
+
+inline dim3 zgrid(const Geom& g, int lpairs, int nb, int nz) {
+  const int npairs = (g.rows + 1) / 2;
+  return dim3((npairs + lpairs - 1) / lpairs, nb, nz);
+}
+
+#define KW_ZDISPATCH(NAME, ...)                                            \
+  switch (g.nz) {                                                          \
+    KW_FOR_SIZES(NAME)                                                     \
+    default: return hipErrorInvalidValue;                                  \
+  }
+
+}  // namespace kw

@@ -10,15 +10,18 @@ constexpr int kMaxBufs = 8;
 
 // Geometry of one plan (every instance in a plan has the same grid shape).
 // Real fields are C-ordered with the last axis contiguous; spectra hold the
-// half spectrum along that axis:  spec[row * nh + k],  k < nh = nz/2 + 1.
+// half spectrum along that axis:  spec[row * nhp + k],  k < nh = nz/2 + 1.
+// The pitch nhp > nh keeps the stride of the outermost-axis pass away from a
+// power of two (measured: 3.3 -> 4.2 TB/s for 512^3 column copies).
 struct Geom {
   int nd;            // 2 or 3
   int n[3];          // axis lengths (n[2] = 1 in 2-D)
   int nz;            // contiguous-axis length n[nd-1]
-  int nh;            // nz / 2 + 1
+  int nh;            // nz / 2 + 1 (spectral columns)
+  int nhp;           // row pitch of a spectrum (nh rounded up to 16 complex)
   int rows;          // ntot / nz
   int64_t ntot;      // real elements per instance
-  int64_t nspec;     // rows * nh complex elements per instance
+  int64_t nspec;     // rows * nhp complex elements per instance (storage)
 };
 
 // One time-varying source term (p, or one velocity component).

@@ -1,0 +1,13 @@
+// Launchers instantiating the time-loop kernels (see kw_kernels.hpp).
+#include "kw_kernels.hpp"
+
+namespace kw {
+
+hipError_t launch_momentum(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st) {
+#define KW_C(S) case S: hipLaunchKernelGGL(k_momentum<S>, zgrid(g, ZCfg<S>::L, nb, g.nd), dim3(ZCfg<S>::NT), 0, st, g, fl, inst, tw); break;
+  KW_ZDISPATCH(KW_C)
+#undef KW_C
+  return hipGetLastError();
+}
+
+}  // namespace kw

@@ -16,7 +16,7 @@ CSRC = PKG.parent / "csrc"
 INCLUDE = PKG.parent.parent / "include"
 OUT_DIR = PKG / "_lib"
 LIB = OUT_DIR / "libkwgpu.so"
-SOURCES = ["kw_kernels.hip", "kw_abi.hip"]
+SOURCES = ["kw_zpass_io.hip", "kw_zpass_mom.hip", "kw_zpass_mass.hip", "kw_zpass_eos.hip", "kw_col.hip", "kw_col_p2.hip", "kw_col_mixed.hip", "kw_misc.hip", "kw_abi.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KW_OFFLOAD_ARCH", "gfx950")

@@ -38,7 +38,7 @@ def _ref(f, spacing, kind, axis, kap, use_sg, power):
     return np.real(np.fft.ifftn(op * np.fft.fftn(f)))
 
 
-SHAPES_2D = [(64, 64), (128, 96), (48, 160), (384, 512), (640, 56), (24, 336)]
+SHAPES_2D = [(64, 64), (128, 96), (48, 160), (384, 512), (640, 56), (24, 448)]
 SHAPES_3D = [(32, 48, 64), (16, 64, 40), (64, 64, 64)]
 
 
