@@ -55,6 +55,8 @@ extern "C" {
 #define KW_F_REC_UX 17  /* read: recorded staggered ux (uy 18, uz 19) */
 #define KW_F_AGG_P 20   /* read: running [max, min, sum of squares] of recorded p, [3][nsens] */
 #define KW_F_STEP 21    /* read: step counter (as float) */
+#define KW_F_ALPHA_NP 22 /* scalar absorption coefficient [Np/((rad/s)^y m)]: tau, eta derived from c0 on the device
+                           (kspace_solver.py:495-498, 522) instead of grid-sized KW_F_TAU / KW_F_ETA */
 
 /* source modes (kspace_solver.py:60-114) */
 #define KW_SRC_DIRICHLET 1

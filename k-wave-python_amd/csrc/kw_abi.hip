@@ -136,6 +136,7 @@ struct kw_plan {
   std::vector<Inst> h_inst;
   std::vector<char> inst_set;
   std::vector<float*> p0;
+  std::vector<char> alpha_scalar;
   Inst* d_inst = nullptr;
   int nspecbuf = 0;       // allocated spectrum buffers
   bool chain_p = false;
@@ -259,7 +260,7 @@ double zbytes_mass(const kw_plan* P) {
     b += 4.0 * nr + 8.0 * ns * (((P->flags & KW_POWERLAW_ABS) ? 1 : 0) + ((P->flags & KW_POWERLAW_DISP) ? 1 : 0));
   } else {
     b += 4.0 * nr + 8.0 * ns + (I.c0sq_s ? 4.0 * nr : 0.0);
-    if ((P->flags & KW_STOKES) && I.tau_s) b += 4.0 * nr;
+    if ((P->flags & KW_STOKES) && I.tau && I.tau_s) b += 4.0 * nr;
     if ((P->flags & KW_NONLINEAR) && I.bona_s) b += 4.0 * nr;
   }
   return b;
@@ -268,8 +269,8 @@ double zbytes_eos(const kw_plan* P) {
   const kw::Inst& I = P->h_inst[0];
   const double nr = (double)P->nb * P->g.ntot, ns = (double)P->nb * P->g.rows * P->g.nh;
   double b = 4.0 * nr + 4.0 * nr + 8.0 * ns;  // rho_total, p, P_z
-  if (P->flags & KW_POWERLAW_ABS) b += 8.0 * ns + (I.tau_s ? 4.0 * nr : 0.0);
-  if (P->flags & KW_POWERLAW_DISP) b += 8.0 * ns + (I.eta_s ? 4.0 * nr : 0.0);
+  if (P->flags & KW_POWERLAW_ABS) b += 8.0 * ns + ((I.tau && I.tau_s) ? 4.0 * nr : 0.0);
+  if (P->flags & KW_POWERLAW_DISP) b += 8.0 * ns + ((I.eta && I.eta_s) ? 4.0 * nr : 0.0);
   if (I.c0sq_s) b += 4.0 * nr;
   if (P->flags & KW_NONLINEAR) b += (I.bona_s ? 4.0 * nr : 0.0) + (I.rho0_s ? 4.0 * nr : 0.0);
   return b;
@@ -419,6 +420,7 @@ int kw_plan_create(const kw_plan_desc* d, kw_plan** out) {
   P->h_inst.assign(P->nb, Inst{});
   P->inst_set.assign(P->nb, 0);
   P->p0.assign(P->nb, nullptr);
+  P->alpha_scalar.assign(P->nb, 0);
   P->nrec.assign(P->nb, 0);
   P->nsens.assign(P->nb, 0);
   P->rec_flags.assign(P->nb, 0);
@@ -468,6 +470,17 @@ int kw_plan_set_field(kw_plan* P, int b, int which, const float* data, int64_t n
     case KW_F_BONA: I.bona = dp; I.bona_s = s; break;
     case KW_F_TAU: I.tau = dp; I.tau_s = s; break;
     case KW_F_ETA: I.eta = dp; I.eta_s = s; break;
+    case KW_F_ALPHA_NP: {
+      if (n != 1) return fail(KW_EINVAL, "alpha (Np) is a scalar field");
+      const double y = P->alpha_power, a_np = data[0];
+      I.tau_c = (float)(-2.0 * a_np);
+      I.eta_c = (float)(2.0 * a_np * std::tan(M_PI * y / 2.0));
+      I.ym1h = (float)((y - 1.0) / 2.0);
+      I.yh = (float)(y / 2.0);
+      I.tau = nullptr; I.eta = nullptr;
+      P->alpha_scalar[b] = 1;
+      break;
+    }
     case KW_F_P0:
       if (n != P->g.ntot) return fail(KW_EINVAL, "p0 must be grid-sized");
       P->p0[b] = dp; break;
@@ -618,8 +631,8 @@ int kw_plan_init(kw_plan* P) {
     for (int ax = 0; ax < g.nd; ++ax)
       if (!I.dtr0[ax]) return fail(KW_EINVAL, "dt/rho0_sg must be set for every axis");
     if ((P->flags & KW_NONLINEAR) && !I.bona) return fail(KW_EINVAL, "B/A must be set for a nonlinear plan");
-    if ((P->flags & (KW_STOKES | KW_POWERLAW_ABS)) && !I.tau) return fail(KW_EINVAL, "tau must be set for an absorbing plan");
-    if ((P->flags & KW_POWERLAW_DISP) && !I.eta) return fail(KW_EINVAL, "eta must be set for a dispersive plan");
+    if ((P->flags & (KW_STOKES | KW_POWERLAW_ABS)) && !I.tau && !P->alpha_scalar[b]) return fail(KW_EINVAL, "tau or alpha must be set for an absorbing plan");
+    if ((P->flags & KW_POWERLAW_DISP) && !I.eta && !P->alpha_scalar[b]) return fail(KW_EINVAL, "eta or alpha must be set for a dispersive plan");
     with_p0 += P->p0[b] != nullptr;
   }
   if (with_p0 != 0 && with_p0 != P->nb) return fail(KW_EINVAL, "either every instance of a plan has p0 or none has");

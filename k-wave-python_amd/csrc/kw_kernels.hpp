@@ -43,6 +43,14 @@ struct ZCfg {
 
 __device__ __forceinline__ float ld(const float* p, int64_t i, int s) { return p[i * s]; }
 
+// absorption / dispersion coefficient at idx given c0^2 there
+__device__ __forceinline__ float tau_at(const Inst& I, int64_t idx, float c2) {
+  return I.tau ? ld(I.tau, idx, I.tau_s) : I.tau_c * exp2f(I.ym1h * log2f(c2));
+}
+__device__ __forceinline__ float eta_at(const Inst& I, int64_t idx, float c2) {
+  return I.eta ? ld(I.eta, idx, I.eta_s) : I.eta_c * exp2f(I.yh * log2f(c2));
+}
+
 // pml value of axis `ax` at grid row r, contiguous position n
 __device__ __forceinline__ float pml_at(const float* const* tab, const Geom& g, int ax, int r, int n) {
   if (ax == g.nd - 1) return tab[ax][n];
@@ -324,9 +332,10 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_mass(Geom g, Fields fl, const I
         a_out[h] = r0 * du;
       } else {
         float q = rtv;
-        if (flags & FLAG_STOKES) q += ld(I.tau, idx, I.tau_s) * r0 * du;
+        const float c2 = ld(I.c0sq, idx, I.c0sq_s);
+        if (flags & FLAG_STOKES) q += tau_at(I, idx, c2) * r0 * du;
         if (flags & FLAG_NONLINEAR) q += ld(I.bona, idx, I.bona_s) * rtv * rtv / (2.f * r0);
-        const float pv = ld(I.c0sq, idx, I.c0sq_s) * q;
+        const float pv = c2 * q;
         fl.p[off + idx] = pv;
         a_out[h] = pv;
       }
@@ -373,11 +382,12 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_eos(Geom g, Fields fl, const In
       if (r >= rows) continue;
       const int64_t idx = (int64_t)r * N + n;
       const float rt = h ? rtv[m].y : rtv[m].x;
+      const float c2 = h ? c2v[m].y : c2v[m].x;
       float q = rt;
-      if (flags & FLAG_POWERLAW_ABS) q += ld(I.tau, idx, I.tau_s) * (h ? av[m].y : av[m].x);
-      if (flags & FLAG_POWERLAW_DISP) q -= ld(I.eta, idx, I.eta_s) * (h ? bv[m].y : bv[m].x);
+      if (flags & FLAG_POWERLAW_ABS) q += tau_at(I, idx, c2) * (h ? av[m].y : av[m].x);
+      if (flags & FLAG_POWERLAW_DISP) q -= eta_at(I, idx, c2) * (h ? bv[m].y : bv[m].x);
       if (flags & FLAG_NONLINEAR) q += ld(I.bona, idx, I.bona_s) * rt * rt / (2.f * ld(I.rho0, idx, I.rho0_s));
-      const float pv = (h ? c2v[m].y : c2v[m].x) * q;
+      const float pv = c2 * q;
       fl.p[off + idx] = pv;
       out[h] = pv;
     }
@@ -570,6 +580,12 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) void k_col(Geom g, Fields f
     if (g.nd == 3) { const float kz = ops.k[2][iz]; kyz2 += kz * kz; }
   }
   float2 st[C::E];
+  float kap_m[C::E];  // kappa(|k|) of each owned element, shared by the gradient/divergence jobs
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const float kx = ops.k[0][j0 + m * C::TPL];
+    kap_m[m] = kap > 0.f ? sinc_k(kap * sqrtf(fmaf(kx, kx, kyz2))) : 1.f;
+  }
   int cur_in = -1;
   for (int j = 0; j < a.njob; ++j) {
     const XJob jb = a.job[j];
@@ -589,8 +605,8 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) void k_col(Geom g, Fields f
       const float km = sqrtf(fmaf(kx, kx, kyz2));
       float2 op;
       switch (jb.kind) {
-        case XK_GRAD: op = cscale(ops.grad[jb.axis][ia], kap > 0.f ? sinc_k(kap * km) : 1.f); break;
-        case XK_DIV: op = cscale(ops.div[jb.axis][ia], kap > 0.f ? sinc_k(kap * km) : 1.f); break;
+        case XK_GRAD: op = cscale(ops.grad[jb.axis][ia], kap_m[m]); break;
+        case XK_DIV: op = cscale(ops.div[jb.axis][ia], kap_m[m]); break;
         case XK_FRAC1: op = make_float2(kpow(km, ops.frac1), 0.f); break;
         case XK_FRAC2: op = make_float2(kpow(km, ops.frac2), 0.f); break;
         case XK_SRCKAPPA: op = make_float2(kap > 0.f ? cos_k(kap * km) : 1.f, 0.f); break;

@@ -51,8 +51,11 @@ struct Inst {
   const float* rho0; int rho0_s;
   const float* dtr0[3]; int dtr0_s;  // dt / rho0_staggered_i
   const float* bona; int bona_s;
-  const float* tau; int tau_s;       // absorption coefficient (grid)
-  const float* eta; int eta_s;       // dispersion coefficient (grid)
+  const float* tau; int tau_s;       // absorption coefficient (grid), or null:
+  const float* eta; int eta_s;       // dispersion coefficient (grid), or null:
+  // with a scalar alpha the coefficients are derived from c0^2 in registers:
+  // tau = tau_c (c0^2)^{(y-1)/2},  eta = eta_c (c0^2)^{y/2}
+  float tau_c, eta_c, ym1h, yh;
   Src srcp;
   Src srcu[3];
   // sensors

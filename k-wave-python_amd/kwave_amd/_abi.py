@@ -14,7 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libkwgpu.so"
 KW_OK, KW_EINVAL, KW_ENOMEM, KW_EHIP, KW_EUNSUPPORTED = 0, -1, -2, -3, -4
 KW_NONLINEAR, KW_STOKES, KW_POWERLAW_ABS, KW_POWERLAW_DISP = 1, 2, 4, 8
 F_C0SQ, F_RHO0, F_DTR0_X, F_BONA, F_TAU, F_ETA, F_P0 = 0, 1, 2, 5, 6, 7, 8
-F_P, F_UX, F_RHOX, F_REC_P, F_REC_UX, F_AGG_P, F_STEP = 9, 10, 13, 16, 17, 20, 21
+F_P, F_UX, F_RHOX, F_REC_P, F_REC_UX, F_AGG_P, F_STEP, F_ALPHA_NP = 9, 10, 13, 16, 17, 20, 21, 22
 SRC_DIRICHLET, SRC_ADDITIVE, SRC_ADDITIVE_NOCORR = 1, 2, 4
 REC_P, REC_P_AGG, REC_UX = 1, 2, 4
 

@@ -73,6 +73,8 @@ class DevicePlan:
             self._field(b, _abi.F_DTR0_X + ax, s.dtr0[ax])
         if s.bona is not None:
             self._field(b, _abi.F_BONA, s.bona)
+        if s.alpha_np is not None and s.flags & (_abi.KW_STOKES | _abi.KW_POWERLAW_ABS | _abi.KW_POWERLAW_DISP):
+            self._field(b, _abi.F_ALPHA_NP, s.alpha_np)
         if s.tau is not None:
             self._field(b, _abi.F_TAU, s.tau)
         if s.eta is not None:

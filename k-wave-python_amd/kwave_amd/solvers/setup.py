@@ -167,6 +167,7 @@ class InstanceSetup:
     alpha_power: float = 1.5
     tau: object = None
     eta: object = None
+    alpha_np: float = None             # scalar alpha: tau/eta derived from c0 on the device
     bona: object = None
     src_p: SourceTerm = None
     src_u: list = field(default_factory=list)
@@ -254,16 +255,21 @@ def build_instance(kgrid, medium, source, sensor, *, use_sg=True, use_kspace=Tru
         a_val, a_scalar = scalar_or_grid(a_raw, shape, "alpha_coeff")
         a_np = db2neper(a_val, y)
         stokes = abs(y - 2.0) < 1e-10
+        if a_scalar:
+            st.alpha_np = float(a_np)
         if mode != "no_absorption":
             if stokes:
                 st.flags |= _abi.KW_STOKES
-                st.tau = -2 * a_np * c_val
+                if not a_scalar:
+                    st.tau = -2 * a_np * c_val
             else:
                 st.flags |= _abi.KW_POWERLAW_ABS
-                st.tau = -2 * a_np * c_val ** (y - 1)
+                if not a_scalar:
+                    st.tau = -2 * a_np * c_val ** (y - 1)
         if mode not in ("no_absorption", "no_dispersion") and not stokes:
             st.flags |= _abi.KW_POWERLAW_DISP
-            st.eta = 2 * a_np * c_val ** y * np.tan(np.pi * y / 2)
+            if not a_scalar:
+                st.eta = 2 * a_np * c_val ** y * np.tan(np.pi * y / 2)
     b_raw = getattr(medium, "BonA", 0)
     if is_enabled(b_raw):
         st.flags |= _abi.KW_NONLINEAR
