@@ -1,0 +1,295 @@
+"""Pins of the FP64 oracle (and of the host set-up helpers) that do not need a GPU.
+
+Each check is chosen so that a plausible mistake in the restatement of
+kwave/solvers/kspace_solver.py fails it:
+  * reference checksum fixtures -> smooth(p0), make_disc/make_ball, _stagger;
+  * reference known answers      -> db2neper, get_optimal_pml_size;
+  * exact closed form            -> k-space operators, kappa, PML-off leapfrog,
+                                    t = 0 initialisation and its u(dt/2) sign;
+  * plane-wave amplitude         -> additive source scaling 2 dt / (D c0 d);
+  * scipy RegularGridInterpolator -> Cartesian sensors;
+  * dirichlet / symmetry / PML invariants.
+"""
+import hashlib
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+GOLDEN = Path(__file__).parent / "golden" / "reference_checksums.json"
+
+
+def _sha_hdf5_layout(a):
+    """tests/h5_summary.py:26-33 applied to an array as the legacy writer stores it (F-order, float32)."""
+    a32 = np.asarray(a, dtype=np.float32)
+    arr = np.ascontiguousarray(a32.T).reshape((1,) * (3 - a32.ndim) + a32.T.shape)
+    return hashlib.sha256(arr.round(8) + 0).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return json.loads(GOLDEN.read_text())
+
+
+def _ivp_2d_inputs(hetero):
+    from kwave_amd.data import Vector
+    from kwave_amd.utils.mapgen import make_disc
+
+    gs = Vector([128, 128])
+    p0 = 5 * make_disc(gs, Vector([50, 50]), 8) + 3 * make_disc(gs, Vector([80, 60]), 5)
+    c = np.full((128, 128), 1500.0)
+    rho = np.full((128, 128), 1000.0)
+    if hetero:
+        c[:64, :] = 1800
+        rho[:, 128 // 4 - 1:] = 1200
+    return p0, c, rho
+
+
+def _ivp_3d_inputs():
+    from kwave_amd.data import Vector
+    from kwave_amd.utils.mapgen import make_ball
+
+    gs = Vector([64, 64, 64])
+    p0 = 10 * make_ball(gs, Vector([38, 32, 32]), 5) + 10 * make_ball(gs, Vector([20, 20, 20]), 3)
+    c = np.full((64, 64, 64), 1500.0)
+    c[:32] = 1800
+    rho = np.full((64, 64, 64), 1000.0)
+    rho[:, 64 // 4 - 1:, :] = 1200
+    return p0, c, rho
+
+
+@pytest.mark.parametrize("impl", ["product", "oracle"])
+def test_smoothed_p0_matches_reference_checksums(golden, impl):
+    from kwave_amd.utils.filters import smooth as smooth_product
+    from oracle.kspace_ref import smooth as smooth_oracle
+
+    smooth = smooth_product if impl == "product" else smooth_oracle
+    for name, hetero in [("out_ivp_homogeneous_medium", False), ("out_ivp_heterogeneous_medium", True)]:
+        p0, _, _ = _ivp_2d_inputs(hetero)
+        assert _sha_hdf5_layout(smooth(p0, restore_max=True)) == golden[name]["p0_source_input"]["sha256_round8"]
+    p0, _, _ = _ivp_3d_inputs()
+    assert _sha_hdf5_layout(smooth(p0, restore_max=True)) == golden["out_ivp_3D_simulation"]["p0_source_input"]["sha256_round8"]
+
+
+@pytest.mark.parametrize("impl", ["product", "oracle"])
+def test_staggered_density_matches_reference_checksums(golden, impl):
+    from kwave_amd.solvers.setup import stagger as stagger_product
+    from oracle.kspace_ref import OracleSimulation
+
+    stagger = stagger_product if impl == "product" else OracleSimulation._stagger
+    _, c, rho = _ivp_2d_inputs(True)
+    g = golden["out_ivp_heterogeneous_medium"]
+    assert _sha_hdf5_layout(c) == g["c0"]["sha256_round8"]
+    assert _sha_hdf5_layout(rho) == g["rho0"]["sha256_round8"]
+    assert _sha_hdf5_layout(stagger(rho, 0)) == g["rho0_sgx"]["sha256_round8"]
+    assert _sha_hdf5_layout(stagger(rho, 1)) == g["rho0_sgy"]["sha256_round8"]
+    _, c3, rho3 = _ivp_3d_inputs()
+    g3 = golden["out_ivp_3D_simulation"]
+    for ax, key in enumerate(["rho0_sgx", "rho0_sgy", "rho0_sgz"]):
+        assert _sha_hdf5_layout(stagger(rho3, ax)) == g3[key]["sha256_round8"]
+    # a plausible mistake (averaging with the previous instead of the next cell) is caught
+    wrong = np.concatenate([rho[:1], 0.5 * (rho[1:] + rho[:-1])], axis=0)
+    assert _sha_hdf5_layout(wrong) != g["rho0_sgx"]["sha256_round8"] or np.allclose(wrong, stagger(rho, 0))
+    wrong_y = np.concatenate([rho[:, :1], 0.5 * (rho[:, 1:] + rho[:, :-1])], axis=1)
+    assert _sha_hdf5_layout(wrong_y) != g["rho0_sgy"]["sha256_round8"]
+
+
+def test_db2neper_known_answer():
+    """tests/test_utils.py:31-35 of the reference."""
+    from kwave_amd.utils.conversion import db2neper, neper2db
+    from oracle.kspace_ref import db2neper as db2neper_oracle
+
+    for f in (db2neper, db2neper_oracle):
+        assert np.isclose(f(1.6), 2.931742395517710e-06)
+        assert np.allclose(f(1.6 * np.ones((10, 10))), 2.931742395517710e-06)
+    assert np.isclose(neper2db(1.5), 8.186258123051049e05)
+
+
+def test_optimal_pml_size_known_answers():
+    """tests/test_pmlutils.py:15-69 of the reference: 64-point axes pick 32."""
+    from kwave_amd.kgrid import kWaveGrid
+    from kwave_amd.utils.pml import get_optimal_pml_size
+
+    for n in ([64], [64, 64], [64, 64, 64]):
+        kg = kWaveGrid(n, [1e-3 / 64] * len(n))
+        assert np.all(get_optimal_pml_size(kg) == 32)
+
+
+def test_pml_profile_closed_form():
+    """kwave/utils/pml.py:43-58: interior exactly 1, outermost cell exp(-alpha c dt/(2 dx)),
+    staggered variant shifted by half a cell; product and oracle agree."""
+    from kwave_amd.utils.pml import get_pml
+    from oracle.kspace_ref import get_pml as get_pml_oracle
+
+    N, dx, dt, c, L, a = 96, 1e-4, 2e-8, 1500.0, 20, 2.0
+    for stag in (False, True):
+        p = get_pml(N, dx, dt, c, L, a, staggered=stag).ravel()
+        o = get_pml_oracle(N, dx, dt, c, L, a, stag)
+        assert np.allclose(p, o, rtol=0, atol=1e-15)
+        assert np.all(p[L:N - L] == 1.0)
+        edge = 1.0 + (0.5 if stag else 0.0)
+        assert np.isclose(p[-1] if not stag else o[-1], np.exp(-a * (c / dx) * ((L + (0.5 if stag else 0)) / L) ** 4 * dt / 2))
+        assert np.isclose(o[0], np.exp(-a * (c / dx) * (((edge) - L - 1) / (-L)) ** 4 * dt / 2))
+
+
+def _closed_form_ivp(p0, c, dt, dx, n):
+    """Exact solution of the PML-free lossless k-space scheme: p^n = F^-1[cos(n c |k| dt) F[p0]]."""
+    ks = [2 * np.pi * np.fft.fftfreq(s, dx) for s in p0.shape]
+    K = np.sqrt(sum(k ** 2 for k in np.meshgrid(*ks, indexing="ij")))
+    return np.real(np.fft.ifftn(np.cos(n * c * K * dt) * np.fft.fftn(p0)))
+
+
+@pytest.mark.parametrize("shape", [(48, 40), (24, 20, 16)])
+def test_oracle_homogeneous_ivp_matches_closed_form(shape):
+    from kwave_amd.kgrid import kWaveGrid
+    from kwave_amd.kmedium import kWaveMedium
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.ksource import kSource
+    from oracle.kspace_ref import OracleSimulation
+
+    rng = np.random.default_rng(0)
+    dx, c = 1e-4, 1500.0
+    dt = 0.3 * dx / c
+    kg = kWaveGrid(list(shape), [dx] * len(shape))
+    kg.setTime(30, dt)
+    src = kSource()
+    p0 = rng.standard_normal(shape)
+    src.p0 = p0
+    sen = kSensor(np.ones(shape, dtype=bool))
+    sim = OracleSimulation(kg, kWaveMedium(sound_speed=c, density=1000.0), src, sen, smooth_p0=False,
+                           pml_size=(0,) * len(shape), pml_alpha=(0.0,) * len(shape))
+    res = sim.run()
+    for n in (0, 1, 7, 29):
+        want = _closed_form_ivp(p0, c, dt, dx, n).ravel()
+        assert np.allclose(res["p"][:, n], want, rtol=0, atol=1e-10 * np.abs(p0).max()), n
+
+
+def test_closed_form_catches_a_flipped_shift():
+    """Mutation check: flipping the staggered shift of op_grad breaks the closed-form agreement."""
+    from kwave_amd.kgrid import kWaveGrid
+    from kwave_amd.kmedium import kWaveMedium
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.ksource import kSource
+    from oracle.kspace_ref import OracleSimulation
+
+    shape, dx, c = (32, 32), 1e-4, 1500.0
+    dt = 0.3 * dx / c
+    kg = kWaveGrid(list(shape), [dx, dx])
+    kg.setTime(10, dt)
+    src = kSource()
+    src.p0 = np.random.default_rng(1).standard_normal(shape)
+    sim = OracleSimulation(kg, kWaveMedium(sound_speed=c), src, kSensor(np.ones(shape, dtype=bool)), smooth_p0=False,
+                           pml_size=(0, 0), pml_alpha=(0.0, 0.0))
+    sim.op_grad = [np.conj(g) * -1 for g in sim.op_grad]  # i k e^{-ik d/2} kappa: the divergence shift
+    res = sim.run()
+    want = _closed_form_ivp(src.p0, c, dt, dx, 9).ravel()
+    assert not np.allclose(res["p"][:, 9], want, atol=1e-6)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("mode", ["additive", "additive-no-correction"])
+def test_additive_plane_source_emits_its_amplitude(ndim, mode):
+    """A full-width additive pressure source emits plane waves whose amplitude equals the
+    source amplitude (the purpose of the 2 dt / (D c0 d) scaling, kspace_solver.py:619-624);
+    a wrong D or factor 2 is off by 2x."""
+    from kwave_amd.kgrid import kWaveGrid
+    from kwave_amd.kmedium import kWaveMedium
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.ksource import kSource
+    from kwave_amd.utils.signals import tone_burst
+    from oracle.kspace_ref import OracleSimulation
+
+    nx, nt = 192, 520
+    shape = (nx,) + (8,) * (ndim - 1)
+    dx, c = 1e-4, 1500.0
+    dt = 0.2 * dx / c
+    kg = kWaveGrid(list(shape), [dx] * ndim)
+    kg.setTime(nt, dt)
+    src = kSource()
+    m = np.zeros(shape, dtype=bool)
+    m[50] = True
+    src.p_mask = m
+    sig = tone_burst(1 / dt, 1e6, 3)
+    src.p = sig
+    src.p_mode = mode
+    sm = np.zeros(shape, dtype=bool)
+    sm[(130,) + (4,) * (ndim - 1)] = True
+    sm[(25,) + (4,) * (ndim - 1)] = True
+    res = OracleSimulation(kg, kWaveMedium(sound_speed=c), src, kSensor(sm), pml_size=(20,) + (0,) * (ndim - 1),
+                           pml_alpha=(2.0,) + (0.0,) * (ndim - 1)).run()
+    for row in res["p"]:
+        assert abs(np.abs(row).max() / np.abs(sig).max() - 1.0) < 0.01
+
+
+def test_dirichlet_source_imposes_the_signal():
+    """Lossless linear medium: a dirichlet pressure source fixes p at the mask to the signal."""
+    from kwave_amd.kgrid import kWaveGrid
+    from kwave_amd.kmedium import kWaveMedium
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.ksource import kSource
+    from oracle.kspace_ref import OracleSimulation
+
+    shape = (48, 48)
+    kg = kWaveGrid(list(shape), [1e-4, 1e-4])
+    kg.setTime(60, 2e-8)
+    m = np.zeros(shape, dtype=bool)
+    m[20, 10:30] = True
+    src = kSource()
+    src.p_mask = m
+    sig = np.sin(np.arange(60) * 0.3)[None, :]
+    src.p = sig
+    src.p_mode = "dirichlet"
+    res = OracleSimulation(kg, kWaveMedium(sound_speed=1500.0, density=1000.0), src, kSensor(m), pml_size=(8, 8),
+                           pml_alpha=(2.0, 2.0)).run()
+    assert np.allclose(res["p"], np.broadcast_to(sig, res["p"].shape), atol=1e-12)
+
+
+def test_cartesian_sensor_interpolation_table_matches_scipy():
+    """Product-side (index, weight) tables reproduce RegularGridInterpolator(linear, bounds_error=True)."""
+    from scipy.interpolate import RegularGridInterpolator
+
+    from kwave_amd.solvers.setup import linear_interp_table
+
+    rng = np.random.default_rng(3)
+    for shape, d in [((40, 34), [1e-4, 2e-4]), ((12, 10, 9), [1e-4, 1e-4, 3e-4])]:
+        f = rng.standard_normal(shape)
+        axes = [(np.arange(n) - n // 2) * dd for n, dd in zip(shape, d)]
+        pts = np.stack([rng.uniform(ax[0], ax[-1], 200) for ax in axes])
+        pts[:, 0] = [ax[0] for ax in axes]
+        pts[:, 1] = [ax[-1] for ax in axes]
+        idx, w = linear_interp_table(pts, shape, d)
+        got = (f.ravel()[idx] * w).sum(axis=1)
+        want = RegularGridInterpolator(axes, f, method="linear", bounds_error=True)(pts.T)
+        assert np.allclose(got, want, rtol=0, atol=1e-12)
+        bad = pts.copy()
+        bad[0, 5] = axes[0][-1] + d[0]
+        with pytest.raises(ValueError):
+            linear_interp_table(bad, shape, d)
+
+
+@pytest.mark.parametrize("use_sg", [False, True])
+def test_oracle_mirror_symmetry(use_sg):
+    """A problem symmetric under y -> -y (about the grid centre) stays symmetric (catches index/sign slips).
+
+    Without a PML: the reference's staggered PML profile is offset by half a cell
+    (kwave/utils/pml.py:46-48) and is applied to the velocity even with
+    use_sg=False, which breaks the mirror symmetry by design."""
+    from kwave_amd.kgrid import kWaveGrid
+    from kwave_amd.kmedium import kWaveMedium
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.ksource import kSource
+    from oracle.kspace_ref import OracleSimulation
+
+    n = 33
+    kg = kWaveGrid([n, n], [1e-4, 1e-4])
+    kg.setTime(40, 2e-8)
+    p0 = np.zeros((n, n))
+    p0[10:14, 14:19] = 1.0
+    src = kSource()
+    src.p0 = p0
+    res = OracleSimulation(kg, kWaveMedium(sound_speed=1500.0, alpha_coeff=0.5, alpha_power=1.5, BonA=5.0), src,
+                           kSensor(np.ones((n, n), dtype=bool)), smooth_p0=False, use_sg=use_sg,
+                           pml_size=(0, 0), pml_alpha=(0.0, 0.0)).run()
+    p = res["p_final"]
+    assert np.allclose(p, p[:, ::-1], atol=1e-12 * np.abs(p).max())
