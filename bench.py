@@ -71,6 +71,42 @@ def cpu_baseline(n=128, steps=5):
     }
 
 
+def scan_benchmark(count=96, seed=4):
+    """Uneven 2-D sweep: `count` scan-line simulations batched vs run one at a time on the same GPU."""
+    import numpy as np
+
+    from kwave_amd import workloads
+    from kwave_amd.batch import BatchSimulation
+    from kwave_amd.solvers.kspace_solver import collect_results
+    from kwave_amd.solvers.plan import DevicePlan
+
+    t0 = time.perf_counter()
+    cases = workloads.scan_sweep_2d(seed=seed, count=count)
+    bs = BatchSimulation(cases)
+    log(f"scan: {count} instances in {len(bs.plans)} size groups, set up in {time.perf_counter() - t0:.1f} s")
+    voxel_steps = sum(int(np.prod(s.shape)) * s.Nt for s in bs.setups)
+    t0 = time.perf_counter()
+    batched = bs.run()
+    t_batched = time.perf_counter() - t0
+    bs.close()
+    log(f"scan: batched {t_batched:.2f} s")
+    singles = [DevicePlan([st]) for st in bs.setups]
+    t0 = time.perf_counter()
+    for pl, st in zip(singles, bs.setups):
+        pl.run(st.Nt - pl.step_index(0))
+    t_looped = time.perf_counter() - t0
+    diff = 0.0
+    for pl, st, res in zip(singles, bs.setups, batched):
+        one = collect_results(pl, 0, st, np.float64)
+        diff = max(diff, float(np.max(np.abs(one["p"] - res["p"])) / max(np.max(np.abs(one["p"])), 1e-30)))
+        pl.close()
+    log(f"scan: looped {t_looped:.2f} s, speedup {t_looped / t_batched:.2f}x, max rel diff {diff:.1e}")
+    return {"instances": count, "size_groups": len(bs.groups), "voxel_steps": voxel_steps,
+            "batched_s": t_batched, "looped_s": t_looped, "speedup_vs_looped": t_looped / t_batched,
+            "sims_per_s": count / t_batched, "voxel_steps_per_s": voxel_steps / t_batched,
+            "max_rel_diff_batched_vs_looped": diff}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +114,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512, help="grid edge (512 is the benchmark workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-scan", action="store_true", help="skip the batched 96-instance 2-D sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         log("this benchmark runs on one GPU only: the 512^3 workload is a single simulation on one MI355X; "
@@ -151,6 +188,12 @@ def main():
         "finite": finite,
     }
     plan.close()
+    if not args.no_scan:
+        try:
+            out["batched_scan"] = scan_benchmark()
+        except Exception as e:
+            log(f"scan benchmark failed: {e!r}")
+            out["batched_scan"] = None
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline()

@@ -55,6 +55,7 @@ extern "C" {
 #define KW_F_REC_UX 17  /* read: recorded staggered ux (uy 18, uz 19) */
 #define KW_F_AGG_P 20   /* read: running [max, min, sum of squares] of recorded p, [3][nsens] */
 #define KW_F_STEP 21    /* read: step counter (as float) */
+#define KW_F_REC_UCX 23 /* read: recorded velocity on the pressure grid (unstaggered, kspace_solver.py:741-743), uy 24, uz 25 */
 #define KW_F_ALPHA_NP 22 /* scalar absorption coefficient [Np/((rad/s)^y m)]: tau, eta derived from c0 on the device
                            (kspace_solver.py:495-498, 522) instead of grid-sized KW_F_TAU / KW_F_ETA */
 
@@ -67,6 +68,7 @@ extern "C" {
 #define KW_REC_P 1
 #define KW_REC_P_AGG 2
 #define KW_REC_UX 4  /* staggered velocity components: UY 8, UZ 16 */
+#define KW_REC_UCX 32 /* velocity shifted to the pressure grid: UCY 64, UCZ 128 */
 
 typedef struct kw_plan kw_plan;
 
@@ -124,6 +126,12 @@ int kw_plan_init(kw_plan* plan);
 /* Advance every active instance by up to nsteps steps (Simulation.step,
  * kspace_solver.py:691-747); instances stop at their own Nt. */
 int kw_plan_run(kw_plan* plan, int64_t nsteps);
+/* Enqueue nsteps steps on the plan's stream and return without waiting, so
+ * that several plans (e.g. the grid-size groups of an uneven batch) run
+ * concurrently; kw_plan_sync waits for them.  The batched counterpart of
+ * calling Simulation.run() per instance (kspace_solver.py:749-761). */
+int kw_plan_launch(kw_plan* plan, int64_t nsteps);
+int kw_plan_sync(kw_plan* plan);
 /* Like kw_plan_run, without graph capture, bracketing every kernel launch with
  * HIP events on the plan stream.  Per kernel kind k (< kw_kernel_kind_count())
  * it returns the summed device time [ms], the algorithmic bytes moved and the

@@ -141,6 +141,7 @@ struct kw_plan {
   int nspecbuf = 0;       // allocated spectrum buffers
   bool chain_p = false;
   bool chain_u[3] = {false, false, false};
+  bool rec_uc[3] = {false, false, false};  // some instance records the collocated velocity
   int max_sens = 0;
   std::vector<int> nrec, nsens;
   std::vector<unsigned> rec_flags;
@@ -303,7 +304,11 @@ int enqueue_step(kw_plan* P) {
   // --- mass: div u, rho update, EOS ---
   jobs.clear();
   std::vector<int> yb;
-  for (int i = 0; i < D; ++i) { jobs.push_back({i, i, XK_DIV, i}); yb.push_back(i); }
+  for (int i = 0; i < D; ++i) {
+    jobs.push_back({i, i, XK_DIV, i});
+    yb.push_back(i);
+    if (P->rec_uc[i]) { jobs.push_back({i, 2 * D + 1 + i, XK_UNSTAGGER, i}); yb.push_back(2 * D + 1 + i); }
+  }
   if (P->chain_p) {
     KW_TLAUNCH(KK_SRC, 0.0, launch_src_build(g, P->nb, P->fl, P->d_inst, twz, -1, P->st));
     jobs.push_back({D, D, XK_SRCKAPPA, 0});
@@ -312,6 +317,9 @@ int enqueue_step(kw_plan* P) {
   if ((rc = ypass(P, 0, yb))) return rc;
   if ((rc = xpass(P, jobs, KK_XDIV))) return rc;
   if ((rc = ypass(P, 1, yb))) return rc;
+  for (int i = 0; i < D; ++i)
+    if (P->rec_uc[i])
+      KW_TLAUNCH(KK_SRC, 0.0, launch_zinv(g, P->nb, P->fl.spec[2 * D + 1 + i], g.nspec, P->fl.uc[i], g.ntot, twz, 1.f, P->st));
   KW_TLAUNCH(KK_MASS, zbytes_mass(P), launch_mass(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st, P->chain_p));
   if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP)) {
     jobs.clear();
@@ -604,9 +612,21 @@ int kw_plan_set_sensor(kw_plan* P, int b, int nsens, int ncorner, const int64_t*
     if ((rc = P->mem.alloc(&I.rec_p, (size_t)nrec * nsens))) return rc;
   if (rec_flags & KW_REC_P_AGG)
     if ((rc = P->mem.alloc(&I.agg_p, (size_t)3 * nsens))) return rc;
-  for (int i = 0; i < P->g.nd; ++i)
+  for (int i = 0; i < P->g.nd; ++i) {
+    I.rec_uc[i] = nullptr;
     if (rec_flags & (KW_REC_UX << i))
       if ((rc = P->mem.alloc(&I.rec_u[i], (size_t)nrec * nsens))) return rc;
+    if (rec_flags & (KW_REC_UCX << i)) {
+      if ((rc = P->mem.alloc(&I.rec_uc[i], (size_t)nrec * nsens))) return rc;
+      if (!P->rec_uc[i]) {
+        if ((rc = ensure_spec(P, 2 * P->g.nd + 1 + i))) return rc;
+        if ((rc = P->mem.alloc(&P->fl.uc[i], (size_t)P->nb * P->g.ntot))) return rc;
+        P->rec_uc[i] = true;
+        if (P->g1) { (void)hipGraphExecDestroy(P->g1); P->g1 = nullptr; }
+        if (P->gk) { (void)hipGraphExecDestroy(P->gk); P->gk = nullptr; }
+      }
+    }
+  }
   P->nrec[b] = nrec;
   P->nsens[b] = nsens;
   P->rec_flags[b] = rec_flags;
@@ -659,10 +679,52 @@ int kw_plan_init(kw_plan* P) {
     if ((rc = xpass(P, jobs, KK_XGRAD))) return rc;
     if ((rc = ypass(P, 1, yb))) return rc;
     KW_LAUNCH(launch_init_u(g, P->nb, P->fl, P->d_inst, twz, P->st));
+    // velocity on the pressure grid for the t = 0 record (kspace_solver.py:741-743)
+    std::vector<XJob> uj;
+    std::vector<int> us;
+    for (int i = 0; i < g.nd; ++i)
+      if (P->rec_uc[i]) {
+        const int s = 2 * g.nd + 1 + i;
+        KW_LAUNCH(launch_zfwd(g, P->nb, P->fl.u[i], g.ntot, P->fl.spec[s], g.nspec, twz, P->st));
+        uj.push_back({s, s, XK_UNSTAGGER, i});
+        us.push_back(s);
+      }
+    if (!uj.empty()) {
+      if ((rc = ypass(P, 0, us))) return rc;
+      if ((rc = xpass(P, uj, KK_XDIV))) return rc;
+      if ((rc = ypass(P, 1, us))) return rc;
+      for (int i = 0; i < g.nd; ++i)
+        if (P->rec_uc[i])
+          KW_LAUNCH(launch_zinv(g, P->nb, P->fl.spec[2 * g.nd + 1 + i], g.nspec, P->fl.uc[i], g.ntot, twz, 1.f, P->st));
+    }
     KW_LAUNCH(launch_zfwd(g, P->nb, P->fl.p, g.ntot, P->fl.spec[0], g.nspec, twz, P->st));
     KW_LAUNCH(launch_gather(g, P->nb, P->max_sens, P->fl, P->d_inst, P->st));
     KW_LAUNCH(launch_tick(P->fl, P->d_inst, P->nb, P->st));
   }
+  KW_HIP(hipStreamSynchronize(P->st));
+  return KW_OK;
+}
+
+int kw_plan_launch(kw_plan* P, int64_t nsteps) {
+  if (!P || nsteps < 0) return fail(KW_EINVAL, "bad plan or step count");
+  KW_HIP(hipSetDevice(P->dev));
+  int rc;
+  if ((rc = sync_inst(P))) return rc;
+  if (P->graphs) {
+    if (!P->g1 && (rc = build_graphs(P))) return rc;
+    int64_t s = 0;
+    for (; s + P->graph_k <= nsteps; s += P->graph_k) KW_HIP(hipGraphLaunch(P->gk, P->st));
+    for (; s < nsteps; ++s) KW_HIP(hipGraphLaunch(P->g1, P->st));
+  } else {
+    for (int64_t s = 0; s < nsteps; ++s)
+      if ((rc = enqueue_step(P))) return rc;
+  }
+  return KW_OK;
+}
+
+int kw_plan_sync(kw_plan* P) {
+  if (!P) return fail(KW_EINVAL, "NULL plan");
+  KW_HIP(hipSetDevice(P->dev));
   KW_HIP(hipStreamSynchronize(P->st));
   return KW_OK;
 }
@@ -716,7 +778,8 @@ int kw_plan_field_size(kw_plan* P, int b, int which, int64_t* n) {
   if (!P || !n || b < 0 || b >= P->nb) return fail(KW_EINVAL, "bad plan or instance");
   const Geom& g = P->g;
   if (which == KW_F_P || (which >= KW_F_UX && which < KW_F_UX + g.nd) || (which >= KW_F_RHOX && which < KW_F_RHOX + g.nd)) *n = g.ntot;
-  else if (which == KW_F_REC_P || (which >= KW_F_REC_UX && which < KW_F_REC_UX + g.nd)) *n = (int64_t)P->nrec[b] * P->nsens[b];
+  else if (which == KW_F_REC_P || (which >= KW_F_REC_UX && which < KW_F_REC_UX + g.nd) ||
+           (which >= KW_F_REC_UCX && which < KW_F_REC_UCX + g.nd)) *n = (int64_t)P->nrec[b] * P->nsens[b];
   else if (which == KW_F_AGG_P) *n = 3 * (int64_t)P->nsens[b];
   else if (which == KW_F_STEP) *n = 1;
   else return fail(KW_EINVAL, "field cannot be read");
@@ -739,6 +802,7 @@ int kw_plan_read(kw_plan* P, int b, int which, float* out, int64_t n) {
   else if (which >= KW_F_RHOX && which < KW_F_RHOX + 3) src = P->fl.rho[which - KW_F_RHOX] + off;
   else if (which == KW_F_REC_P) src = I.rec_p;
   else if (which >= KW_F_REC_UX && which < KW_F_REC_UX + 3) src = I.rec_u[which - KW_F_REC_UX];
+  else if (which >= KW_F_REC_UCX && which < KW_F_REC_UCX + 3) src = I.rec_uc[which - KW_F_REC_UCX];
   else if (which == KW_F_AGG_P) src = I.agg_p;
   else if (which == KW_F_STEP) {
     int t = 0;

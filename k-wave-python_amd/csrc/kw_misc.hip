@@ -33,8 +33,10 @@ __global__ void k_gather(Geom g, Fields fl, const Inst* __restrict__ inst) {
       else { a[s] = fmaxf(a[s], v); a[I.nsens + s] = fminf(a[I.nsens + s], v); a[2 * I.nsens + s] += v * v; }
     }
   }
-  for (int i = 0; i < g.nd; ++i)
+  for (int i = 0; i < g.nd; ++i) {
     if (I.rec_u[i]) I.rec_u[i][row] = sample(fl.u[i]);
+    if (I.rec_uc[i]) I.rec_uc[i][row] = sample(fl.uc[i]);
+  }
 }
 
 __global__ void k_tick(Fields fl, const Inst* __restrict__ inst, int nb) {

@@ -6,7 +6,7 @@
 namespace kw {
 
 constexpr int kMaxJobs = 8;
-constexpr int kMaxBufs = 8;
+constexpr int kMaxBufs = 12;
 
 // Geometry of one plan (every instance in a plan has the same grid shape).
 // Real fields are C-ordered with the last axis contiguous; spectra hold the
@@ -67,6 +67,7 @@ struct Inst {
   int rec_start;        // first recorded step (0-based)
   float* rec_p;         // [nrec][nsens] or null
   float* rec_u[3];      // staggered velocity, [nrec][nsens] or null
+  float* rec_uc[3];     // velocity shifted to the pressure grid, [nrec][nsens] or null
   float* agg_p;         // [3][nsens]: running max, min, sum of squares (or null)
 };
 
@@ -107,6 +108,7 @@ struct Fields {
   float* rho[3];
   float* rhot;
   float2* spec[kMaxBufs];
+  float* uc[3];     // velocity on the pressure grid (only when it is recorded)
   int* tdev;        // per-instance step counter
 };
 

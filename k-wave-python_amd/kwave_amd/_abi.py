@@ -14,16 +14,16 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libkwgpu.so"
 KW_OK, KW_EINVAL, KW_ENOMEM, KW_EHIP, KW_EUNSUPPORTED = 0, -1, -2, -3, -4
 KW_NONLINEAR, KW_STOKES, KW_POWERLAW_ABS, KW_POWERLAW_DISP = 1, 2, 4, 8
 F_C0SQ, F_RHO0, F_DTR0_X, F_BONA, F_TAU, F_ETA, F_P0 = 0, 1, 2, 5, 6, 7, 8
-F_P, F_UX, F_RHOX, F_REC_P, F_REC_UX, F_AGG_P, F_STEP, F_ALPHA_NP = 9, 10, 13, 16, 17, 20, 21, 22
+F_P, F_UX, F_RHOX, F_REC_P, F_REC_UX, F_AGG_P, F_STEP, F_ALPHA_NP, F_REC_UCX = 9, 10, 13, 16, 17, 20, 21, 22, 23
 SRC_DIRICHLET, SRC_ADDITIVE, SRC_ADDITIVE_NOCORR = 1, 2, 4
-REC_P, REC_P_AGG, REC_UX = 1, 2, 4
+REC_P, REC_P_AGG, REC_UX, REC_UCX = 1, 2, 4, 32
 
 # every symbol include/kwgpu.h declares
 EXPORTS = (
     "kw_abi_version", "kw_last_error", "kw_device_count", "kw_size_supported",
     "kw_plan_create", "kw_plan_set_instance", "kw_plan_set_field", "kw_plan_set_source",
     "kw_plan_set_sensor", "kw_plan_init", "kw_plan_run", "kw_plan_read", "kw_plan_field_size",
-    "kw_plan_destroy", "kw_spectral_apply", "kw_plan_run_profiled", "kw_kernel_kind_count", "kw_kernel_kind_name",
+    "kw_plan_destroy", "kw_spectral_apply", "kw_plan_run_profiled", "kw_plan_launch", "kw_plan_sync", "kw_kernel_kind_count", "kw_kernel_kind_name",
 )
 
 
@@ -65,6 +65,8 @@ def lib():
         "kw_plan_set_sensor": ([vp, i32, i32, i32, i64p, f32p, i32, C.c_uint], i32),
         "kw_plan_init": ([vp], i32),
         "kw_plan_run": ([vp, i64], i32),
+        "kw_plan_launch": ([vp, i64], i32),
+        "kw_plan_sync": ([vp], i32),
         "kw_plan_read": ([vp, i32, i32, f32p, i64], i32),
         "kw_plan_field_size": ([vp, i32, i32, C.POINTER(C.c_int64)], i32),
         "kw_plan_destroy": ([vp], i32),

@@ -1,0 +1,4 @@
+"""Reconstruction callers of the solver."""
+from .time_reversal import TimeReversal
+
+__all__ = ["TimeReversal"]

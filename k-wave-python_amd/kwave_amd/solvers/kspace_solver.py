@@ -50,15 +50,37 @@ def compute_aggregates(result, ndim, record):
     return out
 
 
+def acoustic_intensity(result):
+    """Intensity from the recorded series (kspace_solver.py:840-862): the velocity is
+    shifted forward by dt/2 with a temporal Fourier interpolant, then I = p u."""
+    p = result["p"]
+    n = p.shape[-1]
+    freq = 2 * np.pi * np.arange(-(n // 2), n - n // 2) / n
+    shift = np.fft.ifftshift(np.exp(1j * freq * 0.5))
+    out = {}
+    for a in "xyz":
+        u = result.get(f"u{a}")
+        if u is None:
+            continue
+        us = np.real(np.fft.ifft(shift * np.fft.fft(u, axis=-1), axis=-1))
+        out[f"I{a}"] = p * us
+        out[f"I{a}_avg"] = np.mean(out[f"I{a}"], axis=-1)
+    return out
+
+
 def collect_results(plan, b, st, dtype):
     """Assemble the reference's result dict for instance b (kspace_solver.py:762-778)."""
     result = {}
     if "p" in st.record:
         result["p"] = plan.recorded(b, _abi.F_REC_P)
     for i, a in enumerate("xyz"[: st.ndim]):
+        if f"u{a}" in st.record:
+            result[f"u{a}"] = plan.recorded(b, _abi.F_REC_UCX + i)
         if f"u{a}_staggered" in st.record:
             result[f"u{a}_staggered"] = plan.recorded(b, _abi.F_REC_UX + i)
     result.update(compute_aggregates(result, st.ndim, st.record))
+    if "p" in result and any(f"u{a}" in result for a in "xyz") and any(k.startswith("I") for k in st.requested):
+        result.update(acoustic_intensity(result))
     interior = tuple(slice(s, N - s if s else None) for s, N in zip(st.pml_sizes, st.shape))
     if "p_final" in st.requested:
         result["p_final"] = plan.field(b, _abi.F_P)[interior].copy()

@@ -99,9 +99,8 @@ class DevicePlan:
         for i, a in enumerate("xyz"[: s.ndim]):
             if f"u{a}_staggered" in rec:
                 flags |= _abi.REC_UX << i
-        unsupported = [k for k in rec if (k.startswith("u") and not k.endswith("_staggered") and not k.endswith("_final")) or k.startswith("I")]
-        if unsupported:
-            raise NotImplementedError(f"recording {sorted(unsupported)} is not available on the GPU path yet")
+            if f"u{a}" in rec:
+                flags |= _abi.REC_UCX << i
         self._rec_flags.append(flags)
         if flags:
             sidx, sp = _i64(s.sensor_idx)
@@ -112,6 +111,13 @@ class DevicePlan:
     # -- run / read ---------------------------------------------------------
     def run(self, nsteps):
         check(_abi.lib().kw_plan_run(self._h, int(nsteps)))
+
+    def launch(self, nsteps):
+        """Enqueue nsteps steps without waiting (see sync())."""
+        check(_abi.lib().kw_plan_launch(self._h, int(nsteps)))
+
+    def sync(self):
+        check(_abi.lib().kw_plan_sync(self._h))
 
     def run_profiled(self, nsteps):
         """Run nsteps with per-kernel HIP-event timing; returns {kind: (ms, bytes, launches)}."""

@@ -13,8 +13,10 @@ frozen here so results are comparable across runs.
 * ``nonlinear_3d`` 512^3 Gaussian-random c/rho, B/A = 6, power-law absorption,
                   16x16 planar source (0.5 MHz, 4 cycles, 2 MPa), 256x256
                   planar sensor recording p and p_max, 1500 steps.
-Each takes a ``scale`` argument that shrinks the grid and step count
-proportionally for parity tests.
+* ``scan_line_2d`` / ``scan_sweep_2d``  uneven 2-D linear-array sweep
+                  (96 instances of 256^2 .. 640^2, 1200-2600 steps).
+The grid edge ``n`` and step count ``nt`` arguments shrink the problems for
+parity tests.
 """
 from __future__ import annotations
 
@@ -143,3 +145,57 @@ def nonlinear_3d(seed=3, n=512, nt=1500, pml=16):
     sensor = kSensor(smask)
     sensor.record = ["p", "p_max"]
     return kgrid, medium, source, sensor, (pml, pml, pml), (2.0, 2.0, 2.0)
+
+
+SCAN_SIZES = ((256, 256), (384, 512), (512, 768), (640, 640))
+
+
+def scan_line_2d(seed, shape=None, nt=None, pml=20):
+    """One linear-array scan-line simulation of the uneven 2-D sweep.
+
+    Speckle phantom: c = 1540 (1 + U[-0.01, 0.01]) white noise plus 0-4
+    hyperechoic discs (radius U[10, 40] cells, +5 % c), rho = 1000, lossless.
+    Transmit: 32 elements x 2 cells along y at x = pml + 10, 2 MHz 3-cycle tone
+    burst per element, delayed to focus 30 mm deep on the array axis (one signal
+    row per mask point, C order).  Receive: the 32 element centres.  dx = 50 um,
+    Nt uniform in [1200, 2600], PML 20.
+    """
+    rng = np.random.default_rng(seed)
+    if shape is None:
+        shape = SCAN_SIZES[rng.integers(len(SCAN_SIZES))]
+    nx, ny = shape
+    dx = 5e-5
+    c = 1540.0 * (1.0 + rng.uniform(-0.01, 0.01, size=shape))
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    for _ in range(int(rng.integers(0, 5))):
+        r = rng.uniform(10, 40) * min(1.0, min(nx, ny) / 256)
+        cx, cy = rng.uniform(pml + r, nx - pml - r), rng.uniform(pml + r, ny - pml - r)
+        c[(ii - cx) ** 2 + (jj - cy) ** 2 <= r * r] *= 1.05
+    if nt is None:
+        nt = int(rng.integers(1200, 2601))
+    kgrid = kWaveGrid(Vector([nx, ny]), Vector([dx, dx]))
+    dt = 0.3 * dx / c.max()
+    kgrid.setTime(int(nt), dt)
+    medium = kWaveMedium(sound_speed=c, density=1000.0)
+    x0 = pml + 10
+    y_start = ny // 2 - 32
+    mask = np.zeros(shape, dtype=bool)
+    mask[x0, y_start:y_start + 64] = True
+    el_y = (y_start + 2 * np.arange(32) + 0.5) * dx
+    focus = np.array([x0 * dx + 30e-3, (ny // 2) * dx])
+    dist = np.hypot(focus[0] - x0 * dx, focus[1] - el_y)
+    delay = np.round((dist.max() - dist) / 1540.0 / dt).astype(int)
+    burst = tone_burst(1 / dt, 2e6, 3, signal_offset=delay)  # one row per element
+    source = kSource()
+    source.p_mask = mask
+    source.p = 1e5 * np.repeat(burst, 2, axis=0)  # two cells per element, C order along y
+    smask = np.zeros(shape, dtype=bool)
+    smask[x0, y_start + 2 * np.arange(32)] = True
+    sensor = kSensor(smask)
+    sensor.record = ["p"]
+    return kgrid, medium, source, sensor, (pml, pml), (2.0, 2.0)
+
+
+def scan_sweep_2d(seed=4, count=96):
+    """The 96-instance uneven sweep: instance i uses seed 1000 * seed + i."""
+    return [scan_line_2d(1000 * seed + i) for i in range(count)]

@@ -310,3 +310,88 @@ def test_unified_api_pml_outside():
     src2.p0 = smooth(src2.p0, restore_max=True)
     o = _run_oracle(kg2, med2, src2, sen2, (10, 10), (2.0, 2.0), smooth_p0=False)
     _compare(g, o)
+
+
+@pytest.mark.parametrize("shape", [(64, 48), (24, 32, 40)])
+def test_velocity_and_intensity_records(shape):
+    """Collocated velocity (the unstagger FFT pair, kspace_solver.py:741-743), its aggregates,
+    finals and the intensity post-processing (kspace_solver.py:840-862)."""
+    from kwave_amd.kmedium import kWaveMedium
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.ksource import kSource
+
+    kg = _grid(shape, nt=70)
+    c, r = _hetero(shape, 51)
+    med = kWaveMedium(sound_speed=c, density=r, alpha_coeff=0.4, alpha_power=1.4)
+    src = kSource()
+    src.p0 = _p0(shape, 52)
+    mask = np.zeros(shape, dtype=bool)
+    mask[tuple(slice(s // 4, 3 * s // 4, 3) for s in shape)] = True
+    sen = kSensor(mask)
+    ax = "xyz"[: len(shape)]
+    sen.record = ["p", "u", "p_final", "u_final", "ux_max", f"u{ax[-1]}_rms", "I", "I_avg"]
+    pml = tuple([6] * len(shape))
+    g, o = _both((kg, med, src, sen, pml, tuple([2.0] * len(shape))))
+    _compare(g, o)
+
+
+def test_velocity_record_with_time_varying_source():
+    from kwave_amd.kmedium import kWaveMedium
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.ksource import kSource
+
+    shape = (64, 64)
+    kg = _grid(shape, nt=120)
+    med = kWaveMedium(sound_speed=1500.0, density=1000.0)
+    src = kSource()
+    m = np.zeros(shape, dtype=bool)
+    m[20, 20:44] = True
+    src.p_mask = m
+    t = np.arange(60) * float(kg.dt)
+    src.p = 1e3 * np.sin(2 * np.pi * 1e6 * t)[None, :] * np.hanning(60)[None, :]
+    sen = kSensor(np.ones(shape, dtype=bool))
+    sen.record = ["p", "ux", "uy", "uy_min"]
+    g, o = _both((kg, med, src, sen, (8, 8), (2.0, 2.0)))
+    _compare(g, o)
+
+
+def test_time_reversal_reconstruction():
+    """TimeReversal (reconstruction/time_reversal.py) on the GPU vs the same steps on the oracle."""
+    from kwave_amd.kmedium import kWaveMedium
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.ksource import kSource
+    from kwave_amd.kspaceFirstOrder2D import kspaceFirstOrder2D
+    from kwave_amd.options.simulation_execution_options import SimulationExecutionOptions
+    from kwave_amd.options.simulation_options import SimulationOptions
+    from kwave_amd.reconstruction import TimeReversal
+    from oracle.kspace_ref import OracleSimulation
+
+    shape = (64, 64)
+    kg = _grid(shape, nt=150)
+    med = kWaveMedium(sound_speed=1500.0)
+    src = kSource()
+    src.p0 = _p0(shape, 61)
+    mask = np.zeros(shape, dtype=bool)
+    mask[10, 10:54] = mask[53, 10:54] = mask[10:54, 10] = mask[10:54, 53] = True
+    sen = kSensor(mask)
+    sen.record = ["p"]
+    opts = SimulationOptions(pml_size=[8, 8], pml_alpha=2.0)
+    ex = SimulationExecutionOptions(is_gpu_simulation=True, backend="python")
+    import warnings
+
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", FutureWarning)
+        fwd = kspaceFirstOrder2D(kg, copy.deepcopy(src), sen, med, opts, ex)
+        sen.recorded_pressure = fwd["p"]
+        tr = TimeReversal(kg, med, sen)
+        recon = tr(kspaceFirstOrder2D, opts, ex)
+    rsrc = kSource()
+    rsrc.p_mask = mask
+    rsrc.p = np.flip(fwd["p"], axis=1)
+    rsrc.p_mode = "dirichlet"
+    want = OracleSimulation(kg, med, rsrc, kSensor(mask, record=["p", "p_final"]), pml_size=(8, 8), pml_alpha=(2.0, 2.0)).run()
+    want_recon = np.clip(2.0 * want["p_final"], 0, None).T
+    assert recon.shape == want_recon.shape
+    rel = np.linalg.norm(recon - want_recon) / np.linalg.norm(want_recon)
+    print("time reversal rel-L2", rel)
+    assert rel < 1e-4
