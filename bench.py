@@ -33,6 +33,23 @@ HBM_PEAK_GBS = 8000.0
 SURVEY_B_ALG = 412.0 + 60.0 * 440.0 / 1500.0
 
 
+PROFILE = ROOT / "profiles" / "rocprof_3d_nonlinear_512.md"
+# kernel kind (per-kernel HIP-event timing) -> compiled kernel name in the rocprof summary
+ROCPROF_NAME = {"col_y": "kw::k_col<{n}, 16, false, 1>", "z_mass": "kw::k_mass<{n}, 3, false>",
+                "z_momentum": "kw::k_momentum<{n}>", "z_eos": "kw::k_eos<{n}>"}
+
+
+def measured_traffic(kind, n):
+    """Calibrated HBM bytes per dispatch from the committed FETCH_SIZE/WRITE_SIZE summary, or None."""
+    name = ROCPROF_NAME.get(kind, "").format(n=n)
+    if not name or not PROFILE.exists():
+        return None
+    for line in PROFILE.read_text().splitlines():
+        if line.startswith("traffic ") and line[len("traffic "):].rsplit(" ", 1)[0] == name:
+            return float(line.rsplit(" ", 1)[1])
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -105,6 +122,32 @@ def scan_benchmark(count=96, seed=4):
             "batched_s": t_batched, "looped_s": t_looped, "speedup_vs_looped": t_looped / t_batched,
             "sims_per_s": count / t_batched, "voxel_steps_per_s": voxel_steps / t_batched,
             "max_rel_diff_batched_vs_looped": diff}
+
+
+def small_workloads():
+    """Throughput of the two 2-D single-instance workloads (128^2 IVP, 512^2 heterogeneous absorbing)."""
+    import numpy as np
+
+    from kwave_amd import workloads
+    from kwave_amd.solvers.plan import DevicePlan
+    from kwave_amd.solvers.setup import build_instance
+
+    out = {}
+    for name, case in (("2d_ivp_128", workloads.ivp_2d(seed=1)), ("2d_hetero_absorbing_512", workloads.hetero_2d(seed=2))):
+        kg, med, src, sen, pml, alpha = case
+        st = build_instance(kg, med, src, sen, pml_size=pml, pml_alpha=alpha)
+        plan = DevicePlan([st])
+        plan.run(st.Nt - plan.step_index(0))  # warm-up (graph capture)
+        plan.reset()
+        t0 = time.perf_counter()
+        plan.run(st.Nt - plan.step_index(0))
+        wall = time.perf_counter() - t0
+        plan.close()
+        vs = int(np.prod(st.shape)) * st.Nt
+        out[name] = {"grid": list(st.shape), "nt": st.Nt, "wall_s": wall, "ms_per_step": 1e3 * wall / st.Nt,
+                     "voxel_steps_per_s": vs / wall}
+        log(f"{name}: {st.Nt} steps in {wall:.3f} s ({vs / wall:.3e} voxel-steps/s)")
+    return out
 
 
 def main():
@@ -180,7 +223,8 @@ def main():
                    "physics": "heterogeneous c/rho, B/A=6, power-law 0.5 dB/(MHz^1.1 cm)",
                    "source": "16x16 planar additive", "sensor": "256x256 plane, p + p_max"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, n),
+                     "alg_bytes_per_launch": by / nl},
         "hbm_frac_step_model": SURVEY_B_ALG * rate / (HBM_PEAK_GBS * 1e9),
         "alg_bytes_per_voxel_step": alg_bytes_step / ntot,
         "gpu_ms_per_step": gpu_ms_step,
@@ -189,6 +233,10 @@ def main():
     }
     plan.close()
     if not args.no_scan:
+        try:
+            out["small_2d"] = small_workloads()
+        except Exception as e:
+            log(f"2-D workloads failed: {e!r}")
         try:
             out["batched_scan"] = scan_benchmark()
         except Exception as e:

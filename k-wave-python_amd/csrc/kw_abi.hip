@@ -140,6 +140,7 @@ struct kw_plan {
   Inst* d_inst = nullptr;
   int nspecbuf = 0;       // allocated spectrum buffers
   bool chain_p = false;
+  bool pattern_p = false;  // some instance has a single-row additive pressure source
   bool chain_u[3] = {false, false, false};
   bool rec_uc[3] = {false, false, false};  // some instance records the collocated velocity
   int max_sens = 0;
@@ -257,8 +258,10 @@ double zbytes_mass(const kw_plan* P) {
   const double nr = (double)P->nb * P->g.ntot, ns = (double)P->nb * P->g.rows * P->g.nh;
   const int D = P->g.nd;
   double b = 8.0 * ns * D + 8.0 * nr * D + (I.rho0_s ? 4.0 * nr : 0.0);
+  if (P->fl.rhot) b += 4.0 * nr;                      // write the density sum
+  if (P->flags & KW_NONLINEAR) b += 4.0 * nr;         // read the previous sum
   if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP)) {
-    b += 4.0 * nr + 8.0 * ns * (((P->flags & KW_POWERLAW_ABS) ? 1 : 0) + ((P->flags & KW_POWERLAW_DISP) ? 1 : 0));
+    b += 8.0 * ns * (((P->flags & KW_POWERLAW_ABS) ? 1 : 0) + ((P->flags & KW_POWERLAW_DISP) ? 1 : 0));
   } else {
     b += 4.0 * nr + 8.0 * ns + (I.c0sq_s ? 4.0 * nr : 0.0);
     if ((P->flags & KW_STOKES) && I.tau && I.tau_s) b += 4.0 * nr;
@@ -406,7 +409,8 @@ int kw_plan_create(const kw_plan_desc* d, kw_plan** out) {
     if ((rc = P->mem.alloc(&P->fl.u[i], nf))) return cleanup(rc);
     if ((rc = P->mem.alloc(&P->fl.rho[i], nf))) return cleanup(rc);
   }
-  if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP))
+  // sum of the split densities: EOS input (power law) and nonlinear factor of the next step
+  if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP | KW_NONLINEAR))
     if ((rc = P->mem.alloc(&P->fl.rhot, nf))) return cleanup(rc);
   if ((rc = ensure_spec(P, d->ndim - 1))) return cleanup(rc);
   if ((rc = P->mem.alloc(&P->fl.tdev, P->nb))) return cleanup(rc);
@@ -569,6 +573,11 @@ int kw_plan_set_source(kw_plan* P, int b, int which, int mode, int64_t npts, con
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(KW_EHIP, std::string("source pattern: ") + hipGetErrorString(e));
     s.pattern = pat;
+    if (which < 0 && !P->pattern_p) {
+      P->pattern_p = true;
+      if (P->g1) { (void)hipGraphExecDestroy(P->g1); P->g1 = nullptr; }
+      if (P->gk) { (void)hipGraphExecDestroy(P->gk); P->gk = nullptr; }
+    }
   } else {
     s.mode = SRC_CHAIN;
     s.chain_buf = which < 0 ? g.nd : g.nd + 1 + which;

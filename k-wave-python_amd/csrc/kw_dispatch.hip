@@ -1,0 +1,54 @@
+// Size dispatch: each launcher is defined once per size family (f_g0 .. f_g3,
+// see kw_kernels.hpp); the entry points below hand a call to the family that
+// holds the line length.
+#include "kw_launch.hpp"
+
+namespace kw {
+
+#define KW_DECL4(RET, NAME, ...) \
+  RET NAME##_g0(__VA_ARGS__, bool*); RET NAME##_g1(__VA_ARGS__, bool*); RET NAME##_g2(__VA_ARGS__, bool*); RET NAME##_g3(__VA_ARGS__, bool*);
+#define KW_TRY4(NAME, ...)                                    \
+  do {                                                        \
+    bool h = false;                                           \
+    hipError_t e = NAME##_g0(__VA_ARGS__, &h); if (h) return e; \
+    e = NAME##_g1(__VA_ARGS__, &h); if (h) return e;            \
+    e = NAME##_g2(__VA_ARGS__, &h); if (h) return e;            \
+    e = NAME##_g3(__VA_ARGS__, &h); if (h) return e;            \
+    return hipErrorInvalidValue;                              \
+  } while (0)
+
+KW_DECL4(hipError_t, launch_zfwd, const Geom&, int, const float*, int64_t, float2*, int64_t, const float2*, hipStream_t)
+KW_DECL4(hipError_t, launch_zinv, const Geom&, int, const float2*, int64_t, float*, int64_t, const float2*, float, hipStream_t)
+KW_DECL4(hipError_t, launch_init_u, const Geom&, int, const Fields&, const Inst*, const float2*, hipStream_t)
+KW_DECL4(hipError_t, launch_src_build, const Geom&, int, const Fields&, const Inst*, const float2*, int, hipStream_t)
+KW_DECL4(hipError_t, launch_momentum, const Geom&, int, const Fields&, const Inst*, const float2*, hipStream_t)
+KW_DECL4(hipError_t, launch_mass, const Geom&, int, const Fields&, const Inst*, const float2*, int, hipStream_t, bool)
+KW_DECL4(hipError_t, launch_eos, const Geom&, int, const Fields&, const Inst*, const float2*, int, hipStream_t)
+KW_DECL4(hipError_t, launch_col, const Geom&, int, const Fields&, const Ops&, const Inst*, const ColArgs&, int, hipStream_t)
+
+hipError_t launch_zfwd(const Geom& g, int nb, const float* f, int64_t fs, float2* spec, int64_t ss, const float2* tw, hipStream_t st) {
+  KW_TRY4(launch_zfwd, g, nb, f, fs, spec, ss, tw, st);
+}
+hipError_t launch_zinv(const Geom& g, int nb, const float2* spec, int64_t ss, float* f, int64_t fs, const float2* tw, float scale, hipStream_t st) {
+  KW_TRY4(launch_zinv, g, nb, spec, ss, f, fs, tw, scale, st);
+}
+hipError_t launch_init_u(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st) {
+  KW_TRY4(launch_init_u, g, nb, fl, inst, tw, st);
+}
+hipError_t launch_src_build(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int which, hipStream_t st) {
+  KW_TRY4(launch_src_build, g, nb, fl, inst, tw, which, st);
+}
+hipError_t launch_momentum(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st) {
+  KW_TRY4(launch_momentum, g, nb, fl, inst, tw, st);
+}
+hipError_t launch_mass(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int flags, hipStream_t st, bool chain) {
+  KW_TRY4(launch_mass, g, nb, fl, inst, tw, flags, st, chain);
+}
+hipError_t launch_eos(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int flags, hipStream_t st) {
+  KW_TRY4(launch_eos, g, nb, fl, inst, tw, flags, st);
+}
+hipError_t launch_col(const Geom& g, int nb, const Fields& fl, const Ops& ops, const Inst* inst, const ColArgs& a, int nline, hipStream_t st) {
+  KW_TRY4(launch_col, g, nb, fl, ops, inst, a, nline, st);
+}
+
+}  // namespace kw

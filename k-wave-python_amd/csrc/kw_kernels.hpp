@@ -41,7 +41,28 @@ struct ZCfg {
   static constexpr int LDS = Idx::size(L);
 };
 
-__device__ __forceinline__ float ld(const float* p, int64_t i, int s) { return p[i * s]; }
+// Global-memory views of pointers.  Pointers read from parameter blocks are
+// generic to the compiler, and generic (flat) loads also count against the LDS
+// wait counter, so every LDS wait inside a transform would drain them too; the
+// global view keeps them on the vector-memory counter alone.
+#define KW_AS1 __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const KW_AS1 T* gp(const T* p) { return (const KW_AS1 T*)p; }
+template <class T>
+__device__ __forceinline__ KW_AS1 T* gw(T* p) { return (KW_AS1 T*)p; }
+// float2 through a global view, by components (the vector type's members take
+// generic references)
+__device__ __forceinline__ float2 ld2(const KW_AS1 float2* p, int64_t i) {
+  const KW_AS1 float2& v = p[i];
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st2(KW_AS1 float2* p, int64_t i, float2 v) {
+  KW_AS1 float2& d = p[i];
+  d.x = v.x;
+  d.y = v.y;
+}
+
+__device__ __forceinline__ float ld(const float* p, int64_t i, int s) { return gp(p)[i * s]; }
 
 // absorption / dispersion coefficient at idx given c0^2 there
 __device__ __forceinline__ float tau_at(const Inst& I, int64_t idx, float c2) {
@@ -53,22 +74,23 @@ __device__ __forceinline__ float eta_at(const Inst& I, int64_t idx, float c2) {
 
 // pml value of axis `ax` at grid row r, contiguous position n
 __device__ __forceinline__ float pml_at(const float* const* tab, const Geom& g, int ax, int r, int n) {
-  if (ax == g.nd - 1) return tab[ax][n];
-  if (ax == 0) return tab[0][g.nd == 2 ? r : r / g.n[1]];
-  return tab[1][r % g.n[1]];
+  if (ax == g.nd - 1) return gp(tab[ax])[n];
+  if (ax == 0) return gp(tab[0])[g.nd == 2 ? r : r / g.n[1]];
+  return gp(tab[1])[r % g.n[1]];
 }
 
 // Load the half spectra of the L row pairs of this workgroup into LDS as the
 // full spectrum of z = a + i b (Hermitian extension of each half spectrum).
 template <int N>
-__device__ __forceinline__ void load_pair_spec(float2* lds, const float2* __restrict__ spec, int row0, int rows, int pitch, int tid) {
+__device__ __forceinline__ void load_pair_spec(float2* lds, const float2* __restrict__ spec_, int row0, int rows, int pitch, int tid) {
   using C = ZCfg<N>;
+  const auto spec = gp(spec_);
   const typename C::Idx idx{};
   for (int e = tid; e < C::L * C::NH; e += C::NT) {
     const int p = e / C::NH, k = e % C::NH;
     const int ra = row0 + 2 * p, rb = ra + 1;
-    float2 A = ra < rows ? spec[(int64_t)ra * pitch + k] : make_float2(0.f, 0.f);
-    float2 B = rb < rows ? spec[(int64_t)rb * pitch + k] : make_float2(0.f, 0.f);
+    float2 A = ra < rows ? ld2(spec, (int64_t)ra * pitch + k) : make_float2(0.f, 0.f);
+    float2 B = rb < rows ? ld2(spec, (int64_t)rb * pitch + k) : make_float2(0.f, 0.f);
     if (k == 0 || (N % 2 == 0 && k == N / 2)) {
       lds[idx(k, p)] = make_float2(A.x, B.x);
     } else {
@@ -80,16 +102,17 @@ __device__ __forceinline__ void load_pair_spec(float2* lds, const float2* __rest
 
 // After a forward FFT of z = a + i b (in LDS), split into the two half spectra.
 template <int N>
-__device__ __forceinline__ void store_pair_spec(const float2* lds, float2* __restrict__ spec, int row0, int rows, int pitch, int tid) {
+__device__ __forceinline__ void store_pair_spec(const float2* lds, float2* __restrict__ spec_, int row0, int rows, int pitch, int tid) {
   using C = ZCfg<N>;
+  const auto spec = gw(spec_);
   const typename C::Idx idx{};
   for (int e = tid; e < C::L * C::NH; e += C::NT) {
     const int p = e / C::NH, k = e % C::NH;
     const int ra = row0 + 2 * p, rb = ra + 1;
     const float2 Z = lds[idx(k, p)];
     const float2 Zc = lds[idx((N - k) % N, p)];
-    if (ra < rows) spec[(int64_t)ra * pitch + k] = make_float2(0.5f * (Z.x + Zc.x), 0.5f * (Z.y - Zc.y));
-    if (rb < rows) spec[(int64_t)rb * pitch + k] = make_float2(0.5f * (Z.y + Zc.y), -0.5f * (Z.x - Zc.x));
+    if (ra < rows) st2(spec, (int64_t)ra * pitch + k, make_float2(0.5f * (Z.x + Zc.x), 0.5f * (Z.y - Zc.y)));
+    if (rb < rows) st2(spec, (int64_t)rb * pitch + k, make_float2(0.5f * (Z.y + Zc.y), -0.5f * (Z.x - Zc.x)));
   }
 }
 
@@ -127,12 +150,12 @@ __device__ __forceinline__ void fwd_from_regs(float2* lds, float2 (&v)[ZCfg<N>::
 // Source value added/imposed at (row r, position n) for a CSR source term.
 // Returns true when (r, n) is a source point; v receives scale * signal(t).
 __device__ __forceinline__ bool csr_lookup(const Src& s, int r, int n, int t, float& v) {
-  int lo = s.row_ptr[r], hi = s.row_ptr[r + 1];
+  int lo = gp(s.row_ptr)[r], hi = gp(s.row_ptr)[r + 1];
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    const int c = s.col[mid];
+    const int c = gp(s.col)[mid];
     if (c == n) {
-      v = s.scale[mid] * s.sig[(int64_t)s.sigrow[mid] * s.len + t];
+      v = gp(s.scale)[mid] * gp(s.sig)[(int64_t)gp(s.sigrow)[mid] * s.len + t];
       return true;
     }
     if (c < n) lo = mid + 1; else hi = mid;
@@ -140,16 +163,58 @@ __device__ __forceinline__ bool csr_lookup(const Src& s, int r, int n, int t, fl
   return false;
 }
 
-// Apply a source term to field value f at flat index idx (row r, position n),
-// axis factor fac. chain holds the inverse-transformed filtered source (mode CHAIN).
-__device__ __forceinline__ float apply_src(const Src& s, float f, int64_t idx, int r, int n, int t, float fac, float chain) {
-  if (s.mode == SRC_NONE || t >= s.len) return f;
-  if (s.mode == SRC_PATTERN) return f + fac * s.sig[t] * s.pattern[idx];
-  if (s.mode == SRC_CHAIN) return f + fac * chain;
-  float v;
-  if (!csr_lookup(s, r, n, t, v)) return f;
-  return s.mode == SRC_DIRICHLET ? fac * v : f + fac * v;
+// Source terms applied inside the unrolled element loops: the dense filtered
+// pattern (PATTERN, value sig(t) * pattern) and the inverse-transformed
+// multi-row source held in registers (CHAIN, value chv).  mode is SRC_NONE once
+// the signal has ended.  The point-source modes (DIRICHLET, NOCORR) are applied
+// afterwards by a rolled pass over the owned source points (csr_apply), which
+// keeps their lookup out of the unrolled code.
+__device__ __forceinline__ int src_mode_at(const Src& s, int t) { return t < s.len ? s.mode : SRC_NONE; }
+__device__ __forceinline__ float src_inline(const Src& s, int mode, float f, int64_t idx, float fac, float sig_t, float chv) {
+  if (mode == SRC_PATTERN) return f + fac * (sig_t * gp(s.pattern)[idx]);
+  if (mode == SRC_CHAIN) return f + fac * chv;
+  return f;
 }
+__device__ __forceinline__ bool src_is_point(int mode) { return mode == SRC_DIRICHLET || mode == SRC_NOCORR; }
+__device__ __forceinline__ float src_point(int mode, float f, float fac, float v) { return mode == SRC_DIRICHLET ? fac * v : f + fac * v; }
+
+// Register-array element (m, h) with a run-time m (selects, no indexed registers).
+template <int E>
+__device__ __forceinline__ float get_lane(const float2 (&a)[E], int m, int h) {
+  float v = 0.f;
+#pragma unroll
+  for (int k = 0; k < E; ++k)
+    if (k == m) v = h ? a[k].y : a[k].x;
+  return v;
+}
+template <int E>
+__device__ __forceinline__ void set_lane(float2 (&a)[E], int m, int h, float v) {
+#pragma unroll
+  for (int k = 0; k < E; ++k)
+    if (k == m) { if (h) a[k].y = v; else a[k].x = v; }
+}
+
+// Minimum waves per SIMD the register allocator must leave room for (kernels
+// with short register arrays only; 1 = unconstrained).  Measured at 512^3: the
+// mass kernel runs 2668 us with 3 waves (168 VGPRs, a few spills) against 2901 us
+// with 2; the momentum kernel is fastest unconstrained (1904 us at 3 waves, 2808
+// us when forced to 4).
+#ifndef KW_WPE_MOM
+#define KW_WPE_MOM 1
+#endif
+#ifndef KW_WPE_MASS
+#define KW_WPE_MASS 3
+#endif
+#ifndef KW_WPE_EOS
+#define KW_WPE_EOS 1
+#endif
+#ifndef KW_WPE_COL
+#define KW_WPE_COL 1
+#endif
+#ifndef KW_WPE_COLOPS
+#define KW_WPE_COLOPS 1
+#endif
+#define KW_WPE(N, W) __attribute__((amdgpu_waves_per_eu((FftShape<N>::E <= 16 ? (W) : 1))))
 
 #define KW_ZPASS_PROLOGUE                                              \
   using C = ZCfg<N>;                                                   \
@@ -173,7 +238,7 @@ template <int N>
 __global__ __launch_bounds__(ZCfg<N>::NT) void k_zfwd(Geom g, const float* __restrict__ f, int64_t fstride, float2* __restrict__ spec,
                                                       int64_t sstride, const float2* __restrict__ tw_g) {
   KW_ZPASS_PROLOGUE
-  const float* fb = f + b * fstride;
+  const auto fb = gp(f + b * fstride);
   float2 v[C::E];
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
@@ -187,7 +252,7 @@ template <int N>
 __global__ __launch_bounds__(ZCfg<N>::NT) void k_zinv(Geom g, const float2* __restrict__ spec, int64_t sstride, float* __restrict__ f,
                                                       int64_t fstride, const float2* __restrict__ tw_g, float scale) {
   KW_ZPASS_PROLOGUE
-  float* fb = f + b * fstride;
+  const auto fb = gw(f + b * fstride);
   float2 v[C::E];
   inv_to_regs<N>(lds, spec + b * sstride, tw, row0, rows, g.nhp, tid, v);
 #pragma unroll
@@ -198,52 +263,94 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_zinv(Geom g, const float2* __re
   }
 }
 
+// Element addressing shared by the step kernels.  A thread owns positions
+// n = j0 + m * TPL of the two rows ra, ra + 1 of its pair.  Row counts are even,
+// so both rows of a pair are inside the grid or both outside (tail workgroups of
+// small grids); loads use rows clamped into the grid so that the unrolled
+// element loops carry no branches and their loads issue together, and only the
+// stores are guarded.  Values computed for rows outside the grid are never stored.
+#define KW_ROWS_CLAMPED                                                          \
+  const bool inside = ra < rows;                                                 \
+  const int rA = min(ra, rows - 2);                                              \
+  const int64_t iA = (int64_t)rA * N + j0; /* element (m, h): iA + h N + m TPL */
+
+// pml multipliers of the rows of a thread along axis ax (1 for the contiguous axis)
+__device__ __forceinline__ void pml_rows(const float* const* tab, const Geom& g, int ax, int rA, float (&ph)[2]) {
+  if (ax == g.nd - 1) { ph[0] = ph[1] = 1.f; return; }
+  ph[0] = pml_at(tab, g, ax, rA, 0);
+  ph[1] = pml_at(tab, g, ax, rA + 1, 0);
+}
+
 // ---------------------------------------------------------------------------
 // momentum: u_i = pml_sg (pml_sg u_i - dt/rho0_sg * grad_i p) (+ source), then
 // forward transform of u_i along z (for the divergence).  blockIdx.z = axis.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(ZCfg<N>::NT) void k_momentum(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g) {
+__global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MOM) void k_momentum(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g) {
   KW_ZPASS_PROLOGUE
   const int ax = blockIdx.z;
   const Inst& I = inst[b];
-  const int t = fl.tdev[b];
+  const int t = gp(fl.tdev)[b];
   if (t >= I.nt) return;
+  // pml along the contiguous axis, staged next to the twiddles (read after the
+  // first barrier of the transform below)
+  __shared__ float pmlz[N];
+  const bool zax = ax == g.nd - 1;
+  if (zax)
+    for (int i_ = tid; i_ < N; i_ += C::NT) pmlz[i_] = gp(I.pmlsg[ax])[i_];
+  KW_ROWS_CLAMPED
   float2* sp = fl.spec[ax] + b * g.nspec;
-  float* u = fl.u[ax] + b * g.ntot;
+  const auto u = gw(fl.u[ax] + b * g.ntot);
   const Src& src = I.srcu[ax];
+  float ph[2];
+  pml_rows(I.pmlsg, g, ax, rA, ph);
   // operands of the update, fetched before the transform so their latency overlaps it
   float2 uo[C::E], dr[C::E];
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
-    const int n = j0 + m * C::TPL;
-    const int64_t i0 = (int64_t)ra * N + n;
-    const bool h0 = ra < rows, h1 = ra + 1 < rows;
-    uo[m] = make_float2(h0 ? u[i0] : 0.f, h1 ? u[i0 + N] : 0.f);
-    dr[m] = make_float2(h0 ? ld(I.dtr0[ax], i0, I.dtr0_s) : 0.f, h1 ? ld(I.dtr0[ax], i0 + N, I.dtr0_s) : 0.f);
+    const int64_t i0 = iA + m * C::TPL;
+    uo[m] = make_float2(u[i0], u[i0 + N]);
+    dr[m] = make_float2(ld(I.dtr0[ax], i0, I.dtr0_s), ld(I.dtr0[ax], i0 + N, I.dtr0_s));
   }
   float2 gv[C::E];
   inv_to_regs<N>(lds, sp, tw, row0, rows, g.nhp, tid, gv);
   float2 ch[C::E];
-  const bool chain = src.mode == SRC_CHAIN && t < src.len;
+  const int smode = src_mode_at(src, t);
+  const bool chain = smode == SRC_CHAIN;
   if (chain) inv_to_regs<N>(lds, fl.spec[src.chain_buf] + b * g.nspec, tw, row0, rows, g.nhp, tid, ch);
+  const float sig_t = smode == SRC_PATTERN ? gp(src.sig)[t] : 0.f;
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
-    const int n = j0 + m * C::TPL;
-    float res[2] = {0.f, 0.f};
+    const float sz = zax ? pmlz[j0 + m * C::TPL] : 1.f;
+    float res[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int r = ra + h;
-      if (r >= rows) continue;
-      const int64_t idx = (int64_t)r * N + n;
-      const float s = pml_at(I.pmlsg, g, ax, r, n);
+      const int64_t idx = iA + h * N + m * C::TPL;
+      const float s = sz * ph[h];
       const float gval = h ? gv[m].y : gv[m].x;
       float uv = s * (s * (h ? uo[m].y : uo[m].x) - (h ? dr[m].y : dr[m].x) * gval);
-      uv = apply_src(src, uv, idx, r, n, t, src.fac[ax], chain ? (h ? ch[m].y : ch[m].x) : 0.f);
-      u[idx] = uv;
+      uv = src_inline(src, smode, uv, idx, src.fac[ax], sig_t, chain ? (h ? ch[m].y : ch[m].x) : 0.f);
       res[h] = uv;
     }
+    if (inside) {
+      u[iA + m * C::TPL] = res[0];
+      u[iA + N + m * C::TPL] = res[1];
+    }
     gv[m] = make_float2(res[0], res[1]);
+  }
+  if (src_is_point(smode) && inside) {
+    for (int h = 0; h < 2; ++h) {
+      const int r = ra + h;
+      if (gp(src.row_ptr)[r] == gp(src.row_ptr)[r + 1]) continue;
+      for (int m = 0; m < C::E; ++m) {
+        const int n = j0 + m * C::TPL;
+        float v;
+        if (!csr_lookup(src, r, n, t, v)) continue;
+        const float uv = src_point(smode, get_lane(gv, m, h), src.fac[ax], v);
+        u[(int64_t)r * N + n] = uv;
+        set_lane(gv, m, h, uv);
+      }
+    }
   }
   fwd_from_regs<N>(lds, gv, sp, tw, row0, rows, g.nhp, tid);
 }
@@ -252,38 +359,38 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_momentum(Geom g, Fields fl, con
 // mass conservation (+ equation of state when no fractional Laplacian is needed)
 // ---------------------------------------------------------------------------
 template <int N, int ND, bool CHAIN>
-__global__ __launch_bounds__(ZCfg<N>::NT) void k_mass(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g, int flags) {
+__global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MASS) void k_mass(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g, int flags) {
   KW_ZPASS_PROLOGUE
   const Inst& I = inst[b];
-  const int t = fl.tdev[b];
+  const int t = gp(fl.tdev)[b];
   if (t >= I.nt) return;
+  __shared__ float pmlz[N];  // pml along the contiguous axis (read after the first transform barrier)
+  for (int i_ = tid; i_ < N; i_ += C::NT) pmlz[i_] = gp(I.pml[ND - 1])[i_];
+  KW_ROWS_CLAMPED
   const int64_t off = b * g.ntot;
   const Src& src = I.srcp;
-  const bool chain = CHAIN && src.mode == SRC_CHAIN && t < src.len;
+  const int smode = src_mode_at(src, t);
+  const bool chain = CHAIN && smode == SRC_CHAIN;
+  const float sig_t = smode == SRC_PATTERN ? gp(src.sig)[t] : 0.f;
   const bool powerlaw = flags & (FLAG_POWERLAW_ABS | FLAG_POWERLAW_DISP);
-  // per owned element and row: rho0, the nonlinear factor (from the old split
-  // densities), and the running sums of div u and of the new densities
-  float2 r0v[C::E], nlv[C::E], divu[C::E], rt[C::E];
+  float ph[ND][2];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) pml_rows(I.pml, g, i, rA, ph[i]);
+  // per owned element and row: the coefficient dt (2 rho_old + rho0) of the
+  // density update (dt rho0 when linear; rho_old = sum of the split densities
+  // of the previous step, kept in rhot), and the running sums of div u and of
+  // the new densities
+  float2 qv[C::E], divu[C::E], rt[C::E];
   float2 ch[CHAIN ? C::E : 1];
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
-    const int n = j0 + m * C::TPL;
-    float r0h[2] = {1.f, 1.f}, nlh[2] = {1.f, 1.f};
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int r = ra + h;
-      if (r >= rows) continue;
-      const int64_t idx = (int64_t)r * N + n;
-      r0h[h] = ld(I.rho0, idx, I.rho0_s);
-      if (flags & FLAG_NONLINEAR) {
-        float rs = 0.f;
-#pragma unroll
-        for (int i = 0; i < ND; ++i) rs += fl.rho[i][off + idx];
-        nlh[h] = (2.f * rs + r0h[h]) / r0h[h];
-      }
+    const int64_t i0 = iA + m * C::TPL;
+    float qa = ld(I.rho0, i0, I.rho0_s), qb = ld(I.rho0, i0 + N, I.rho0_s);
+    if (flags & FLAG_NONLINEAR) {
+      qa += 2.f * gw(fl.rhot)[off + i0];
+      qb += 2.f * gw(fl.rhot)[off + i0 + N];
     }
-    r0v[m] = make_float2(r0h[0], r0h[1]);
-    nlv[m] = make_float2(nlh[0], nlh[1]);
+    qv[m] = make_float2(I.dt * qa, I.dt * qb);
     divu[m] = make_float2(0.f, 0.f);
     rt[m] = make_float2(0.f, 0.f);
   }
@@ -292,52 +399,98 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_mass(Geom g, Fields fl, const I
   }
 #pragma unroll
   for (int i = 0; i < ND; ++i) {
+    const auto rho = gw(fl.rho[i] + off);
+    // old split density, fetched before the transform so its latency overlaps it
+    // (for short register arrays; the long mixed-radix ones load it in place)
+    constexpr bool PF = C::E <= 16;
+    float2 ro[PF ? C::E : 1];
+    if constexpr (PF) {
+#pragma unroll
+      for (int m = 0; m < C::E; ++m) ro[m] = make_float2(rho[iA + m * C::TPL], rho[iA + N + m * C::TPL]);
+    }
     float2 d[C::E];
     inv_to_regs<N>(lds, fl.spec[i] + b * g.nspec, tw, row0, rows, g.nhp, tid, d);
 #pragma unroll
     for (int m = 0; m < C::E; ++m) {
-      const int n = j0 + m * C::TPL;
+      const float sz = (i == ND - 1) ? pmlz[j0 + m * C::TPL] : 1.f;
+      float res[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int r = ra + h;
-        if (r >= rows) continue;
-        const int64_t idx = (int64_t)r * N + n;
+        const int64_t idx = iA + h * N + m * C::TPL;
         const float dv = h ? d[m].y : d[m].x;
-        const float s = pml_at(I.pml, g, i, r, n);
-        const float r0 = h ? r0v[m].y : r0v[m].x;
-        const float nl = h ? nlv[m].y : nlv[m].x;
-        float rv = s * (s * fl.rho[i][off + idx] - I.dt * r0 * dv * nl);
+        const float s = sz * ph[i][h];
+        const float q = h ? qv[m].y : qv[m].x;
+        float rold;
+        if constexpr (PF) rold = h ? ro[m].y : ro[m].x;
+        else rold = rho[idx];
+        float rv = s * (s * rold - q * dv);
         float chv = 0.f;
         if constexpr (CHAIN) chv = chain ? (h ? ch[m].y : ch[m].x) : 0.f;
-        rv = apply_src(src, rv, idx, r, n, t, src.fac[i], chv);
-        fl.rho[i][off + idx] = rv;
-        if (h) { divu[m].y += dv; rt[m].y += rv; } else { divu[m].x += dv; rt[m].x += rv; }
+        rv = src_inline(src, smode, rv, idx, src.fac[i], sig_t, chv);
+        res[h] = rv;
+      }
+      if (inside) {
+        rho[iA + m * C::TPL] = res[0];
+        rho[iA + N + m * C::TPL] = res[1];
+      }
+      divu[m].x += d[m].x;
+      divu[m].y += d[m].y;
+      rt[m].x += res[0];
+      rt[m].y += res[1];
+    }
+  }
+  if (src_is_point(smode) && inside) {
+    // point sources: update the owned source points of every split density and
+    // rebuild their density sums (same order of additions as the loop above)
+    for (int h = 0; h < 2; ++h) {
+      const int r = ra + h;
+      if (gp(src.row_ptr)[r] == gp(src.row_ptr)[r + 1]) continue;
+      for (int m = 0; m < C::E; ++m) {
+        const int n = j0 + m * C::TPL;
+        float v;
+        if (!csr_lookup(src, r, n, t, v)) continue;
+        const int64_t idx = off + (int64_t)r * N + n;
+        float sum = 0.f;
+#pragma unroll
+        for (int i = 0; i < ND; ++i) {
+          const auto rho = gw(fl.rho[i]);
+          const float x = src_point(smode, rho[idx], src.fac[i], v);
+          rho[idx] = x;
+          sum += x;
+        }
+        set_lane(rt, m, h, sum);
       }
     }
   }
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
-    const int n = j0 + m * C::TPL;
-    float a_out[2] = {0.f, 0.f};
+    float a_out[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int r = ra + h;
-      if (r >= rows) continue;
-      const int64_t idx = (int64_t)r * N + n;
-      const float r0 = h ? r0v[m].y : r0v[m].x;
+      const int64_t idx = iA + h * N + m * C::TPL;
+      // rho0 again: this block read the same lines at its start, so they come from L2
+      const float r0 = ld(I.rho0, idx, I.rho0_s);
       const float du = h ? divu[m].y : divu[m].x;
       const float rtv = h ? rt[m].y : rt[m].x;
       if (powerlaw) {
-        fl.rhot[off + idx] = rtv;
         a_out[h] = r0 * du;
       } else {
         float q = rtv;
         const float c2 = ld(I.c0sq, idx, I.c0sq_s);
         if (flags & FLAG_STOKES) q += tau_at(I, idx, c2) * r0 * du;
         if (flags & FLAG_NONLINEAR) q += ld(I.bona, idx, I.bona_s) * rtv * rtv / (2.f * r0);
-        const float pv = c2 * q;
-        fl.p[off + idx] = pv;
-        a_out[h] = pv;
+        a_out[h] = c2 * q;
+      }
+    }
+    if (inside) {
+      const int64_t i0 = off + iA + m * C::TPL;
+      if (fl.rhot) {
+        gw(fl.rhot)[i0] = rt[m].x;
+        gw(fl.rhot)[i0 + N] = rt[m].y;
+      }
+      if (!powerlaw) {
+        gw(fl.p)[i0] = a_out[0];
+        gw(fl.p)[i0 + N] = a_out[1];
       }
     }
     divu[m] = make_float2(a_out[0], a_out[1]);
@@ -355,41 +508,41 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_mass(Geom g, Fields fl, const I
 // p = c0^2 (rho + tau F^-1[|k|^{y-2} F[rho0 div u]] - eta F^-1[|k|^{y-1} F[rho]] + B/A rho^2 / (2 rho0))
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(ZCfg<N>::NT) void k_eos(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g, int flags) {
+__global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_EOS) void k_eos(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g, int flags) {
   KW_ZPASS_PROLOGUE
   const Inst& I = inst[b];
-  const int t = fl.tdev[b];
+  const int t = gp(fl.tdev)[b];
   if (t >= I.nt) return;
+  KW_ROWS_CLAMPED
   const int64_t off = b * g.ntot;
+  const auto rhot = gw(fl.rhot + off);
   float2 rtv[C::E], c2v[C::E];
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
-    const int64_t i0 = (int64_t)ra * N + j0 + m * C::TPL;
-    const bool h0 = ra < rows, h1 = ra + 1 < rows;
-    rtv[m] = make_float2(h0 ? fl.rhot[off + i0] : 0.f, h1 ? fl.rhot[off + i0 + N] : 0.f);
-    c2v[m] = make_float2(h0 ? ld(I.c0sq, i0, I.c0sq_s) : 0.f, h1 ? ld(I.c0sq, i0 + N, I.c0sq_s) : 0.f);
+    const int64_t i0 = iA + m * C::TPL;
+    rtv[m] = make_float2(rhot[i0], rhot[i0 + N]);
+    c2v[m] = make_float2(ld(I.c0sq, i0, I.c0sq_s), ld(I.c0sq, i0 + N, I.c0sq_s));
   }
   float2 av[C::E], bv[C::E];
   if (flags & FLAG_POWERLAW_ABS) inv_to_regs<N>(lds, fl.spec[0] + b * g.nspec, tw, row0, rows, g.nhp, tid, av);
   if (flags & FLAG_POWERLAW_DISP) inv_to_regs<N>(lds, fl.spec[1] + b * g.nspec, tw, row0, rows, g.nhp, tid, bv);
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
-    const int n = j0 + m * C::TPL;
-    float out[2] = {0.f, 0.f};
+    float out[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int r = ra + h;
-      if (r >= rows) continue;
-      const int64_t idx = (int64_t)r * N + n;
+      const int64_t idx = iA + h * N + m * C::TPL;
       const float rt = h ? rtv[m].y : rtv[m].x;
       const float c2 = h ? c2v[m].y : c2v[m].x;
       float q = rt;
       if (flags & FLAG_POWERLAW_ABS) q += tau_at(I, idx, c2) * (h ? av[m].y : av[m].x);
       if (flags & FLAG_POWERLAW_DISP) q -= eta_at(I, idx, c2) * (h ? bv[m].y : bv[m].x);
       if (flags & FLAG_NONLINEAR) q += ld(I.bona, idx, I.bona_s) * rt * rt / (2.f * ld(I.rho0, idx, I.rho0_s));
-      const float pv = c2 * q;
-      fl.p[off + idx] = pv;
-      out[h] = pv;
+      out[h] = c2 * q;
+    }
+    if (inside) {
+      gw(fl.p)[off + iA + m * C::TPL] = out[0];
+      gw(fl.p)[off + iA + N + m * C::TPL] = out[1];
     }
     av[m] = make_float2(out[0], out[1]);
   }
@@ -415,10 +568,12 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_init_u(Geom g, Fields fl, const
       const int r = ra + h;
       if (r >= rows) continue;
       const int64_t idx = (int64_t)r * N + n;
-      fl.u[ax][off + idx] = 0.5f * ld(I.dtr0[ax], idx, I.dtr0_s) * (h ? gv[m].y : gv[m].x);
+      gw(fl.u[ax])[off + idx] = 0.5f * ld(I.dtr0[ax], idx, I.dtr0_s) * (h ? gv[m].y : gv[m].x);
       if (ax == 0) {
-        const float rv = fl.p[off + idx] / (ld(I.c0sq, idx, I.c0sq_s) * (float)g.nd);
-        for (int j = 0; j < g.nd; ++j) fl.rho[j][off + idx] = rv;
+        const float rv = gw(fl.p)[off + idx] / (ld(I.c0sq, idx, I.c0sq_s) * (float)g.nd);
+        float rs = 0.f;
+        for (int j = 0; j < g.nd; ++j) { gw(fl.rho[j])[off + idx] = rv; rs += rv; }
+        if (fl.rhot) gw(fl.rhot)[off + idx] = rs;
       }
     }
   }
@@ -432,18 +587,18 @@ template <int N>
 __global__ __launch_bounds__(ZCfg<N>::NT) void k_src_build(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g, int which) {
   KW_ZPASS_PROLOGUE
   const Inst& I = inst[b];
-  const int t = fl.tdev[b];
+  const int t = gp(fl.tdev)[b];
   const Src& s = which < 0 ? I.srcp : I.srcu[which];
   if (t >= I.nt || s.mode != SRC_CHAIN || t >= s.len) return;
-  float2* sp = fl.spec[s.chain_buf] + b * g.nspec;
+  const auto sp = gw(fl.spec[s.chain_buf] + b * g.nspec);
   const int rend = min(row0 + 2 * C::L, rows);
-  const int e0 = s.row_ptr[row0], e1 = s.row_ptr[rend];
+  const int e0 = gp(s.row_ptr)[row0], e1 = gp(s.row_ptr)[rend];
   if (e0 == e1) {
     for (int e = tid; e < C::L * C::NH; e += C::NT) {
       const int p = e / C::NH, k = e % C::NH;
       const int r0 = row0 + 2 * p;
-      if (r0 < rows) sp[(int64_t)r0 * g.nhp + k] = make_float2(0.f, 0.f);
-      if (r0 + 1 < rows) sp[(int64_t)(r0 + 1) * g.nhp + k] = make_float2(0.f, 0.f);
+      if (r0 < rows) st2(sp, (int64_t)r0 * g.nhp + k, make_float2(0.f, 0.f));
+      if (r0 + 1 < rows) st2(sp, (int64_t)(r0 + 1) * g.nhp + k, make_float2(0.f, 0.f));
     }
     return;
   }
@@ -453,16 +608,16 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_src_build(Geom g, Fields fl, co
   float* fb = reinterpret_cast<float*>(lds);
   for (int e = e0 + tid; e < e1; e += C::NT) {
     int r = row0;
-    while (s.row_ptr[r + 1] <= e) ++r;
+    while (gp(s.row_ptr)[r + 1] <= e) ++r;
     const int p = (r - row0) >> 1, h = (r - row0) & 1;
-    fb[2 * idx(s.col[e], p) + h] = s.scale[e] * s.sig[(int64_t)s.sigrow[e] * s.len + t];
+    fb[2 * idx(gp(s.col)[e], p) + h] = gp(s.scale)[e] * gp(s.sig)[(int64_t)gp(s.sigrow)[e] * s.len + t];
   }
   __syncthreads();
   float2 v[C::E];
 #pragma unroll
   for (int m = 0; m < C::E; ++m) v[m] = lds[idx(j0 + m * C::TPL, pc)];
   __syncthreads();
-  fwd_from_regs<N>(lds, v, sp, tw, row0, rows, g.nhp, tid);
+  fwd_from_regs<N>(lds, v, fl.spec[s.chain_buf] + b * g.nspec, tw, row0, rows, g.nhp, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -520,17 +675,22 @@ template <int N>
 constexpr int col_lbig() { return clampi(pow2floor(8192 / N), 1, 64); }
 template <int N>
 constexpr int col_lsmall() { return col_lbig<N>() / 4 < 1 ? 1 : col_lbig<N>() / 4; }
+// Tiles of the outermost-axis pass with spectral jobs: half as wide, since that
+// kernel holds more registers (at 512: 942 vs 1015 us for the gradient pass,
+// 1197 vs 1335 us for the divergence pass).
+template <int N>
+constexpr int col_lops() { return col_lbig<N>() / 2 < 1 ? 1 : col_lbig<N>() / 2; }
 
 template <int N, int L, bool OPS, int EM = 1>
-__global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) void k_col(Geom g, Fields fl, Ops ops, const Inst* __restrict__ inst, ColArgs a) {
+__global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COLOPS : KW_WPE_COL)) void k_col(Geom g, Fields fl, Ops ops, const Inst* __restrict__ inst, ColArgs a) {
   using C = ColCfg<N, L, EM>;
   __shared__ float2 lds[C::LDS];
   __shared__ float2 twl[N];
   const typename C::Idx idx{};
   const int tid = threadIdx.x;
   const int b = blockIdx.y;
-  if (fl.tdev && fl.tdev[b] >= inst[b].nt) return;
-  for (int i_ = tid; i_ < N; i_ += C::NT) twl[i_] = ops.tw[a.axis][i_];
+  if (fl.tdev && gp(fl.tdev)[b] >= inst[b].nt) return;
+  for (int i_ = tid; i_ < N; i_ += C::NT) twl[i_] = ld2(gp(ops.tw[a.axis]), i_);
   __syncthreads();
   const int ntile = (a.ncols + L - 1) / L;
   const int lid = xcd_remap(blockIdx.x, gridDim.x);
@@ -547,24 +707,24 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) void k_col(Geom g, Fields f
     // while buffer q is transformed
     float2 nx[C::E];
     if (valid) {
-      const float2* bp = fl.spec[a.bufs[0]];
+      const auto bp = gp(fl.spec[a.bufs[0]]);
 #pragma unroll
-      for (int m = 0; m < C::E; ++m) nx[m] = bp[base + (int64_t)(j0 + m * C::TPL) * a.estride];
+      for (int m = 0; m < C::E; ++m) nx[m] = ld2(bp, base + (int64_t)(j0 + m * C::TPL) * a.estride);
     }
     for (int q = 0; q < a.nbuf; ++q) {
-      float2* bp = fl.spec[a.bufs[q]];
+      const auto bp = gw(fl.spec[a.bufs[q]]);
 #pragma unroll
       for (int m = 0; m < C::E; ++m) v[m] = valid ? nx[m] : make_float2(0.f, 0.f);
       if (q + 1 < a.nbuf && valid) {
-        const float2* bn = fl.spec[a.bufs[q + 1]];
+        const auto bn = gp(fl.spec[a.bufs[q + 1]]);
 #pragma unroll
-        for (int m = 0; m < C::E; ++m) nx[m] = bn[base + (int64_t)(j0 + m * C::TPL) * a.estride];
+        for (int m = 0; m < C::E; ++m) nx[m] = ld2(bn, base + (int64_t)(j0 + m * C::TPL) * a.estride);
       }
       if (a.mode == 0) fft_reg<N, -1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
       else fft_reg<N, +1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
       if (valid) {
 #pragma unroll
-        for (int m = 0; m < C::E; ++m) bp[base + (int64_t)(j0 + m * C::TPL) * a.estride] = v[m];
+        for (int m = 0; m < C::E; ++m) st2(bp, base + (int64_t)(j0 + m * C::TPL) * a.estride, v[m]);
       }
     }
   } else {
@@ -575,15 +735,15 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) void k_col(Geom g, Fields f
   else { iy = o; iz = valid ? col : 0; }
   float kyz2 = 0.f;
   {
-    const float ky = ops.k[1][iy];
+    const float ky = gp(ops.k[1])[iy];
     kyz2 = ky * ky;
-    if (g.nd == 3) { const float kz = ops.k[2][iz]; kyz2 += kz * kz; }
+    if (g.nd == 3) { const float kz = gp(ops.k[2])[iz]; kyz2 += kz * kz; }
   }
   float2 st[C::E];
   float kap_m[C::E];  // kappa(|k|) of each owned element, shared by the gradient/divergence jobs
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
-    const float kx = ops.k[0][j0 + m * C::TPL];
+    const float kx = gp(ops.k[0])[j0 + m * C::TPL];
     kap_m[m] = kap > 0.f ? sinc_k(kap * sqrtf(fmaf(kx, kx, kyz2))) : 1.f;
   }
   int cur_in = -1;
@@ -591,9 +751,9 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) void k_col(Geom g, Fields f
     const XJob jb = a.job[j];
     if (jb.in != cur_in) {
       cur_in = jb.in;
-      const float2* src = fl.spec[jb.in];
+      const auto src = gp(fl.spec[jb.in]);
 #pragma unroll
-      for (int m = 0; m < C::E; ++m) st[m] = valid ? src[base + (int64_t)(j0 + m * C::TPL) * a.estride] : make_float2(0.f, 0.f);
+      for (int m = 0; m < C::E; ++m) st[m] = valid ? ld2(src, base + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
       fft_reg<N, -1, typename C::Idx, EM>(st, lds, tw, c, j0, idx);
     }
     const int ixa = jb.axis == 0 ? -1 : (jb.axis == 1 ? iy : iz);
@@ -601,31 +761,45 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) void k_col(Geom g, Fields f
     for (int m = 0; m < C::E; ++m) {
       const int n = j0 + m * C::TPL;
       const int ia = ixa < 0 ? n : ixa;
-      const float kx = ops.k[0][n];
+      const float kx = gp(ops.k[0])[n];
       const float km = sqrtf(fmaf(kx, kx, kyz2));
       float2 op;
       switch (jb.kind) {
-        case XK_GRAD: op = cscale(ops.grad[jb.axis][ia], kap_m[m]); break;
-        case XK_DIV: op = cscale(ops.div[jb.axis][ia], kap_m[m]); break;
+        case XK_GRAD: op = cscale(ld2(gp(ops.grad[jb.axis]), ia), kap_m[m]); break;
+        case XK_DIV: op = cscale(ld2(gp(ops.div[jb.axis]), ia), kap_m[m]); break;
         case XK_FRAC1: op = make_float2(kpow(km, ops.frac1), 0.f); break;
         case XK_FRAC2: op = make_float2(kpow(km, ops.frac2), 0.f); break;
         case XK_SRCKAPPA: op = make_float2(kap > 0.f ? cos_k(kap * km) : 1.f, 0.f); break;
-        case XK_UNSTAGGER: op = ops.unst[jb.axis][ia]; break;
+        case XK_UNSTAGGER: op = ld2(gp(ops.unst[jb.axis]), ia); break;
         default: op = make_float2(1.f, 0.f); break;
       }
       v[m] = cscale(cmul(st[m], op), ops.inv_ntot);
     }
     fft_reg<N, +1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
     if (valid) {
-      float2* dst = fl.spec[jb.out];
+      const auto dst = gw(fl.spec[jb.out]);
 #pragma unroll
-      for (int m = 0; m < C::E; ++m) dst[base + (int64_t)(j0 + m * C::TPL) * a.estride] = v[m];
+      for (int m = 0; m < C::E; ++m) st2(dst, base + (int64_t)(j0 + m * C::TPL) * a.estride, v[m]);
     }
   }
   }
 }
 
-#define KW_FOR_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(24) X(48) X(96) X(192) X(384) X(768) X(40) X(80) X(160) X(320) X(640) X(56) X(112) X(224) X(448)
+// Supported line lengths, in four radix families.  Launcher files are compiled
+// once per family (-DKW_GROUP=0..3) so the many template instantiations build
+// in parallel; KW_GROUP_SIZES lists the family of the current translation unit
+// and KW_GNAME(f) names its launcher f_g<family>.
+#define KW_SIZES_G0(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
+#define KW_SIZES_G1(X) X(24) X(48) X(96) X(192) X(384) X(768)
+#define KW_SIZES_G2(X) X(40) X(80) X(160) X(320) X(640)
+#define KW_SIZES_G3(X) X(56) X(112) X(224) X(448)
+#define KW_FOR_SIZES(X) KW_SIZES_G0(X) KW_SIZES_G1(X) KW_SIZES_G2(X) KW_SIZES_G3(X)
+#define KW_CAT_(a, b) a##b
+#define KW_CAT(a, b) KW_CAT_(a, b)
+#ifdef KW_GROUP
+#define KW_GROUP_SIZES KW_CAT(KW_SIZES_G, KW_GROUP)
+#define KW_GNAME(f) KW_CAT(f##_g, KW_GROUP)
+#endif
 
 inline bool size_supported_impl(int n) {
   switch (n) {
@@ -641,10 +815,12 @@ inline dim3 zgrid(const Geom& g, int lpairs, int nb, int nz) {
   return dim3((npairs + lpairs - 1) / lpairs, nb, nz);
 }
 
-#define KW_ZDISPATCH(NAME, ...)                                            \
+// switch over the current family's sizes; a size outside it leaves *handled false
+#define KW_ZDISPATCH(NAME)                                                 \
+  *handled = true;                                                         \
   switch (g.nz) {                                                          \
-    KW_FOR_SIZES(NAME)                                                     \
-    default: return hipErrorInvalidValue;                                  \
+    KW_GROUP_SIZES(NAME)                                                   \
+    default: *handled = false; return hipSuccess;                          \
   }
 
 }  // namespace kw

@@ -1,4 +1,4 @@
-// Host-side launchers for the time-loop kernels (defined in kw_kernels.hip).
+// Host-side launchers for the time-loop kernels (size dispatch in kw_dispatch.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>

@@ -1,9 +1,9 @@
-// Launchers instantiating the time-loop kernels (see kw_kernels.hpp).
+// Mass-conservation launcher of one size family (compiled once per family, -DKW_GROUP=0..3).
 #include "kw_kernels.hpp"
 
 namespace kw {
 
-hipError_t launch_mass(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int flags, hipStream_t st, bool chain) {
+hipError_t KW_GNAME(launch_mass)(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int flags, hipStream_t st, bool chain, bool* handled) {
   if (g.nd == 2) {
 #define KW_C(S) case S: if (chain) hipLaunchKernelGGL((k_mass<S, 2, true>), zgrid(g, ZCfg<S>::L, nb, 1), dim3(ZCfg<S>::NT), 0, st, g, fl, inst, tw, flags); else hipLaunchKernelGGL((k_mass<S, 2, false>), zgrid(g, ZCfg<S>::L, nb, 1), dim3(ZCfg<S>::NT), 0, st, g, fl, inst, tw, flags); break;
     KW_ZDISPATCH(KW_C)

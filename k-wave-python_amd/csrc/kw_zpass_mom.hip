@@ -1,9 +1,9 @@
-// Launchers instantiating the time-loop kernels (see kw_kernels.hpp).
+// Momentum launcher of one size family (compiled once per family, -DKW_GROUP=0..3).
 #include "kw_kernels.hpp"
 
 namespace kw {
 
-hipError_t launch_momentum(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st) {
+hipError_t KW_GNAME(launch_momentum)(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st, bool* handled) {
 #define KW_C(S) case S: hipLaunchKernelGGL(k_momentum<S>, zgrid(g, ZCfg<S>::L, nb, g.nd), dim3(ZCfg<S>::NT), 0, st, g, fl, inst, tw); break;
   KW_ZDISPATCH(KW_C)
 #undef KW_C

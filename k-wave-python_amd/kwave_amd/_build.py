@@ -16,7 +16,10 @@ CSRC = PKG.parent / "csrc"
 INCLUDE = PKG.parent.parent / "include"
 OUT_DIR = PKG / "_lib"
 LIB = OUT_DIR / "libkwgpu.so"
-SOURCES = ["kw_zpass_io.hip", "kw_zpass_mom.hip", "kw_zpass_mass.hip", "kw_zpass_eos.hip", "kw_col.hip", "kw_col_p2.hip", "kw_col_mixed.hip", "kw_misc.hip", "kw_abi.hip"]
+# launcher files compiled once per line-length family (-DKW_GROUP=g), heaviest first
+GROUPED = ["kw_zpass_mass.hip", "kw_col_pass.hip", "kw_zpass_mom.hip", "kw_zpass_eos.hip", "kw_zpass_io.hip"]
+GROUPS = (3, 2, 1, 0)
+SOURCES = ["kw_dispatch.hip", "kw_misc.hip", "kw_abi.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KW_OFFLOAD_ARCH", "gfx950")
@@ -32,8 +35,8 @@ FLAGS = [
 ]
 
 
-def _compile(src: Path, obj: Path):
-    cmd = [HIPCC, *FLAGS, "-c", str(src), "-o", str(obj)]
+def _compile(src: Path, obj: Path, defines=()):
+    cmd = [HIPCC, *FLAGS, *defines, "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stdout}\n{r.stderr}")
@@ -46,8 +49,10 @@ def build(verbose: bool = True) -> Path:
     build_dir.mkdir(exist_ok=True)
     OUT_DIR.mkdir(exist_ok=True)
     objs = []
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        futs = [ex.submit(_compile, CSRC / s, build_dir / (Path(s).stem + ".o")) for s in SOURCES]
+    jobs = [(CSRC / s, build_dir / f"{Path(s).stem}.g{g}.o", (f"-DKW_GROUP={g}",)) for g in GROUPS for s in GROUPED]
+    jobs += [(CSRC / s, build_dir / (Path(s).stem + ".o"), ()) for s in SOURCES]
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        futs = [ex.submit(_compile, *j) for j in jobs]
         for f in futs:
             objs.append(f.result())
     tmp = LIB.with_suffix(".so.tmp")
