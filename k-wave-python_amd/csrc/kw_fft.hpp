@@ -40,12 +40,20 @@ struct RadixPlan {
   int r[kMaxStages];
 };
 
+// Powers of two use radix 8 (E = 8).  With an odd factor f the elements per
+// thread are lcm(power-of-two radix, f), so the power-of-two part uses radix 4
+// next to 3 (E = 12) and radix 2 next to 5 or 7 (E = 10, 14): a few more LDS
+// exchanges, but register arrays short enough for several waves per SIMD (radix
+// 8 gave E = 24 / 40 / 56 and single-wave kernels that spilled).
 constexpr RadixPlan make_plan(int n) {
   RadixPlan p{n, 0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int odd = n;
+  while (odd > 1 && odd % 2 == 0) odd /= 2;
+  const int p2max = odd == 1 ? 8 : (odd % 5 == 0 || odd % 7 == 0) ? 2 : 4;
   int m = n;
-  while (m % 8 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 8; m /= 8; }
-  if (m % 4 == 0) { p.r[p.ns++] = 4; m /= 4; }
-  if (m % 2 == 0) { p.r[p.ns++] = 2; m /= 2; }
+  while (p2max >= 8 && m % 8 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 8; m /= 8; }
+  while (p2max >= 4 && m % 4 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 4; m /= 4; }
+  while (m % 2 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 2; m /= 2; }
   while (m % 3 == 0) { p.r[p.ns++] = 3; m /= 3; }
   while (m % 5 == 0) { p.r[p.ns++] = 5; m /= 5; }
   while (m % 7 == 0) { p.r[p.ns++] = 7; m /= 7; }

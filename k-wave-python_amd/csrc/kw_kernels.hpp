@@ -145,6 +145,9 @@ __device__ __forceinline__ void fwd_from_regs(float2* lds, float2 (&v)[ZCfg<N>::
   for (int m = 0; m < C::E; ++m) lds[idx(j0 + m * C::TPL, c)] = v[m];
   __syncthreads();
   store_pair_spec<N>(lds, spec, row0, rows, pitch, tid);
+  // leave the buffer free: a following transform writes it in its first stage
+  // while slower waves could still be reading the pair spectra here
+  __syncthreads();
 }
 
 // Source value added/imposed at (row r, position n) for a CSR source term.
@@ -214,7 +217,7 @@ __device__ __forceinline__ void set_lane(float2 (&a)[E], int m, int h, float v) 
 #ifndef KW_WPE_COLOPS
 #define KW_WPE_COLOPS 1
 #endif
-#define KW_WPE(N, W) __attribute__((amdgpu_waves_per_eu((FftShape<N>::E <= 16 ? (W) : 1))))
+#define KW_WPE(N, W) __attribute__((amdgpu_waves_per_eu((FftShape<N>::E <= 8 ? (W) : 1))))
 
 #define KW_ZPASS_PROLOGUE                                              \
   using C = ZCfg<N>;                                                   \
