@@ -1,0 +1,52 @@
+"""Run-to-run determinism of the HIP solver.
+
+Two runs of the same simulation must give bit-identical outputs: the solver
+has no atomics and a fixed reduction order, so any difference means a data
+race (for example between LDS phases of consecutive transforms in one
+workgroup).  The shapes give workgroups of two waves (length 80) and four
+waves (length 512, as in the 512^3 benchmark) on the power-law + nonlinear
+path, which runs two forward transforms back to back.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(shape, nt=80):
+    from kwave_amd.data import Vector
+    from kwave_amd.kgrid import kWaveGrid
+    from kwave_amd.kmedium import kWaveMedium
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.ksource import kSource
+
+    rng = np.random.default_rng(21)
+    dx = 1e-4
+    kg = kWaveGrid(Vector(list(shape)), Vector([dx] * len(shape)))
+    kg.setTime(nt, 0.3 * dx / 1700.0)
+    c = 1500.0 + 200.0 * rng.random(shape)
+    med = kWaveMedium(sound_speed=c, density=1000.0 + 100.0 * rng.random(shape), alpha_coeff=0.75, alpha_power=1.5, BonA=6.0)
+    src = kSource()
+    p0 = np.zeros(shape)
+    p0[tuple(s // 2 for s in shape)] = 1e5
+    src.p0 = p0
+    mask = np.zeros(shape, dtype=bool)
+    mask[(slice(None),) * (len(shape) - 1) + (shape[-1] // 3,)] = True
+    sen = kSensor(mask)
+    sen.record = ["p", "p_final"]
+    return kg, med, src, sen
+
+
+@pytest.mark.parametrize("shape", [(48, 80), (16, 512), (24, 32, 40)])
+def test_repeat_runs_bit_identical(shape):
+    from kwave_amd.solvers.kspace_solver import Simulation
+
+    kg, med, src, sen = _case(shape)
+    pml = tuple(4 for _ in shape)
+    outs = [Simulation(kg, med, copy.deepcopy(src), sen, pml_size=pml, pml_alpha=tuple(2.0 for _ in shape)).run() for _ in range(3)]
+    for k in outs[0]:
+        for o in outs[1:]:
+            assert np.array_equal(outs[0][k], o[k]), (shape, k, float(np.max(np.abs(outs[0][k] - o[k]))))
+    assert np.all(np.isfinite(outs[0]["p"]))
