@@ -147,6 +147,21 @@ int kw_plan_field_size(kw_plan* plan, int b, int which, int64_t* n);
 /* Release every device resource of the plan. */
 int kw_plan_destroy(kw_plan* plan);
 
+/* Gradient of a functional J of the recorded pressure traces with respect to
+ * the sound speed c0 of every instance, by the discrete adjoint of the step
+ * (reverse time, forward state checkpointed every `ckpt` steps and replayed
+ * one segment at a time).  adj = dJ/d(recorded p), [batch][nrec][nsens] (the
+ * device record layout); grad_c: [batch][grid size] out.  The gradient holds
+ * c_ref, dt, the PML and kappa fixed and includes the dependence of the
+ * additive source scale 2 dt / (D c0 d) on c0 at the source points.  Linear
+ * lossless plans with additive (or uncorrected additive) pressure sources, no
+ * p0 and no velocity sources; every instance records p with a common Nt,
+ * sensor count and record start.  The plan's fields and records afterwards are
+ * those of the forward run's first segment replay: call kw_plan_init before
+ * stepping it again.  The reference has no adjoint: this differentiates its
+ * Simulation.step (kspace_solver.py:691-747) and source scaling (:619-624). */
+int kw_plan_adjoint_c(kw_plan* plan, const float* adj, int ckpt, float* grad_c);
+
 /* Spectral operator f -> real(ifftn(op * fftn(f))) on the device, the
  * reference's Simulation._diff (kspace_solver.py:781-796), for a single field.
  * kind: 0 identity, 1 gradient (op_grad), 2 divergence (op_div),

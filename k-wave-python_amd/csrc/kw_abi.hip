@@ -2,6 +2,7 @@
 // FP32), step sequencing and hipGraph capture.  Declared in include/kwgpu.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdlib>
@@ -149,6 +150,7 @@ struct kw_plan {
   hipGraphExec_t g1 = nullptr, gk = nullptr;
   int graph_k = 8;
   bool graphs = true;
+  bool skip_gather = false;  // replayed steps of the reverse sweep do not record
   bool inst_dirty = true;
   // per-kernel timing (kw_plan_run_profiled)
   bool prof = false;
@@ -334,7 +336,7 @@ int enqueue_step(kw_plan* P) {
     if ((rc = ypass(P, 1, yb))) return rc;
     KW_TLAUNCH(KK_EOS, zbytes_eos(P), launch_eos(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st));
   }
-  KW_TLAUNCH(KK_GATHER, 0.0, launch_gather(g, P->nb, P->max_sens, P->fl, P->d_inst, P->st));
+  if (!P->skip_gather) KW_TLAUNCH(KK_GATHER, 0.0, launch_gather(g, P->nb, P->max_sens, P->fl, P->d_inst, P->st));
   KW_TLAUNCH(KK_TICK, 0.0, launch_tick(P->fl, P->d_inst, P->nb, P->st));
   return KW_OK;
 }
@@ -821,6 +823,147 @@ int kw_plan_read(kw_plan* P, int b, int which, float* out, int64_t n) {
   }
   if (!src) return fail(KW_EINVAL, "field was not recorded");
   KW_HIP(hipMemcpy(out, src, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return KW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// gradient of a sensor-trace functional with respect to c0 (kw_adjoint.hpp)
+// ---------------------------------------------------------------------------
+namespace {
+
+int adjoint_step(kw_plan* P, const AdjFields& af, float* Zbuf, int t) {
+  const Geom& g = P->g;
+  const int D = g.nd;
+  const float2* twz = P->ops.tw[D - 1];
+  int rc;
+  KW_LAUNCH(launch_adj_scatter(g, P->nb, P->max_sens, af, P->d_inst, t, P->st));
+  KW_LAUNCH(launch_adj_eos(g, P->nb, P->fl, af, P->d_inst, twz, t, P->st));
+  bool src_on = false, filt = false;
+  for (int b = 0; b < P->nb; ++b) {
+    const kw::Src& s = P->h_inst[b].srcp;
+    if (s.mode != SRC_NONE && t < s.len) {
+      src_on = true;
+      filt = filt || s.mode == SRC_PATTERN || s.mode == SRC_CHAIN;
+    }
+  }
+  if (src_on) {
+    if (filt) {  // Z = F^-1[kappa_s F[R]] (K is symmetric)
+      KW_LAUNCH(launch_zfwd(g, P->nb, af.R, g.ntot, P->fl.spec[D], g.nspec, twz, P->st));
+      if ((rc = ypass(P, 0, {D}))) return rc;
+      if ((rc = xpass(P, {{D, D, XK_SRCKAPPA, 0}}, KK_XDIV))) return rc;
+      if ((rc = ypass(P, 1, {D}))) return rc;
+      KW_LAUNCH(launch_zinv(g, P->nb, P->fl.spec[D], g.nspec, Zbuf, g.ntot, twz, 1.f, P->st));
+    }
+    KW_LAUNCH(launch_adj_srcgrad(g, P->nb, af, Zbuf, P->d_inst, t, P->st));
+  }
+  std::vector<int> bufs;
+  std::vector<XJob> jobs;
+  for (int i = 0; i < D; ++i) { bufs.push_back(i); jobs.push_back({i, i, XK_GRAD, i}); }
+  if ((rc = ypass(P, 0, bufs))) return rc;
+  if ((rc = xpass(P, jobs, KK_XGRAD))) return rc;
+  if ((rc = ypass(P, 1, bufs))) return rc;
+  KW_LAUNCH(launch_adj_mom(g, P->nb, P->fl, af, P->d_inst, twz, P->st));
+  for (int i = 0; i < D; ++i) jobs[i].kind = XK_DIV;
+  if ((rc = ypass(P, 0, bufs))) return rc;
+  if ((rc = xpass(P, jobs, KK_XDIV))) return rc;
+  if ((rc = ypass(P, 1, bufs))) return rc;
+  KW_LAUNCH(launch_adj_sum(g, P->nb, P->fl, af, P->d_inst, twz, P->st));
+  return KW_OK;
+}
+
+}  // namespace
+
+int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
+  if (!P || !adj || !grad_c || ckpt < 1) return fail(KW_EINVAL, "NULL argument or checkpoint interval < 1");
+  if (P->flags) return fail(KW_EUNSUPPORTED, "adjoint: linear lossless plans only");
+  const Geom& g = P->g;
+  const int D = g.nd;
+  const int nb = P->nb;
+  const kw::Inst& I0 = P->h_inst[0];
+  for (int b = 0; b < nb; ++b) {
+    if (!P->inst_set[b]) return fail(KW_EINVAL, "instance " + std::to_string(b) + " not set");
+    const kw::Inst& I = P->h_inst[b];
+    if (P->p0[b]) return fail(KW_EUNSUPPORTED, "adjoint: initial pressure is not differentiated");
+    for (int i = 0; i < D; ++i)
+      if (I.srcu[i].mode != SRC_NONE) return fail(KW_EUNSUPPORTED, "adjoint: velocity sources are not differentiated");
+    if (I.srcp.mode == SRC_DIRICHLET) return fail(KW_EUNSUPPORTED, "adjoint: dirichlet pressure sources are not differentiated");
+    if (!I.rec_p) return fail(KW_EINVAL, "adjoint: every instance must record p");
+    if (I.nt != I0.nt || I.nsens != I0.nsens || I.rec_start != I0.rec_start)
+      return fail(KW_EINVAL, "adjoint: instances must share Nt, sensor count and record start");
+  }
+  KW_HIP(hipSetDevice(P->dev));
+  int rc;
+  if ((rc = ensure_spec(P, D))) return rc;
+  if ((rc = sync_inst(P))) return rc;
+  const int nt = I0.nt;
+  const int nrec = nt - I0.rec_start;
+  const int nck = (nt + ckpt - 1) / ckpt;
+  const int64_t nf = (int64_t)nb * g.ntot;
+  const int nstate = 2 * D + 1;  // p, u_i, rho_i
+  DevBuf tmp;                    // freed when the call returns
+  float *ck, *pbuf, *Zbuf, *dadj;
+  AdjFields af{};
+  if ((rc = tmp.alloc(&ck, (size_t)nck * nstate * nf)) || (rc = tmp.alloc(&pbuf, (size_t)std::min(ckpt, nt) * nf)) ||
+      (rc = tmp.alloc(&af.pb, (size_t)nf)) || (rc = tmp.alloc(&af.cb, (size_t)nf)) || (rc = tmp.alloc(&af.R, (size_t)nf)) ||
+      (rc = tmp.alloc(&Zbuf, (size_t)nf)) || (rc = tmp.upload(&dadj, adj, (size_t)nb * nrec * I0.nsens)))
+    return rc;
+  for (int i = 0; i < D; ++i)
+    if ((rc = tmp.alloc(&af.ub[i], (size_t)nf)) || (rc = tmp.alloc(&af.rb[i], (size_t)nf))) return rc;
+  af.adj = dadj;
+  af.nrec = nrec;
+  af.nsens = I0.nsens;
+  std::vector<float*> state = {P->fl.p};
+  for (int i = 0; i < D; ++i) state.push_back(P->fl.u[i]);
+  for (int i = 0; i < D; ++i) state.push_back(P->fl.rho[i]);
+  auto copy_state = [&](int k, bool save) -> int {
+    for (int s = 0; s < nstate; ++s) {
+      float* slot = ck + ((int64_t)k * nstate + s) * nf;
+      KW_HIP(hipMemcpyAsync(save ? slot : state[s], save ? state[s] : slot, sizeof(float) * nf, hipMemcpyDeviceToDevice, P->st));
+    }
+    return KW_OK;
+  };
+  // forward pass: records the traces, keeps the state before every ckpt-th step
+  if ((rc = kw_plan_init(P))) return rc;
+  P->skip_gather = false;
+  for (int t = 0; t < nt; ++t) {
+    if (t % ckpt == 0 && (rc = copy_state(t / ckpt, true))) return rc;
+    if ((rc = enqueue_step(P))) return rc;
+  }
+  // reverse sweep, one checkpoint segment at a time
+  for (float* f : {af.pb, af.cb, af.R}) KW_LAUNCH(launch_fill(f, nf, 0.f, P->st));
+  for (int i = 0; i < D; ++i) {
+    KW_LAUNCH(launch_fill(af.ub[i], nf, 0.f, P->st));
+    KW_LAUNCH(launch_fill(af.rb[i], nf, 0.f, P->st));
+  }
+  int* tdev = P->fl.tdev;
+  P->skip_gather = true;
+  for (int seg = nck - 1; seg >= 0 && rc == KW_OK; --seg) {
+    const int t0 = seg * ckpt, t1 = std::min(nt, t0 + ckpt);
+    P->fl.tdev = tdev;
+    if ((rc = copy_state(seg, false))) break;
+    // a step starts from F_z[p], which the previous step's last kernel leaves in
+    // spectrum buffer 0: rebuild it from the restored p (same transform code)
+    if (launch_zfwd(g, nb, P->fl.p, g.ntot, P->fl.spec[0], g.nspec, P->ops.tw[D - 1], P->st) != hipSuccess) {
+      rc = fail(KW_EHIP, "adjoint: transform of the restored pressure failed");
+      break;
+    }
+    if (launch_setint(tdev, nb, t0, P->st) != hipSuccess) { rc = fail(KW_EHIP, "adjoint: step counter reset failed"); break; }
+    for (int t = t0; t < t1 && rc == KW_OK; ++t) {
+      if ((rc = enqueue_step(P))) break;
+      if (hipMemcpyAsync(pbuf + (int64_t)(t - t0) * nf, P->fl.p, sizeof(float) * nf, hipMemcpyDeviceToDevice, P->st) != hipSuccess)
+        rc = fail(KW_EHIP, "adjoint: copy of p failed");
+    }
+    P->fl.tdev = nullptr;  // column passes of the reverse step must not skip instances at t = Nt
+    for (int t = t1 - 1; t >= t0 && rc == KW_OK; --t) {
+      af.pt = pbuf + (int64_t)(t - t0) * nf;
+      rc = adjoint_step(P, af, Zbuf, t);
+    }
+  }
+  P->fl.tdev = tdev;
+  P->skip_gather = false;
+  if (rc) { (void)hipStreamSynchronize(P->st); return rc; }
+  KW_HIP(hipMemcpyAsync(grad_c, af.cb, sizeof(float) * nf, hipMemcpyDeviceToHost, P->st));
+  KW_HIP(hipStreamSynchronize(P->st));
   return KW_OK;
 }
 

@@ -1,0 +1,164 @@
+// Reverse-time (discrete adjoint) kernels of the linear lossless step, for the
+// gradient of a sensor-trace functional with respect to the sound speed.
+//
+// Forward step t (kw_kernels.hpp; reference kspace_solver.py:691-747):
+//   u_i = a_i^2 u_i - a_i beta_i Gop_i p,   rho_i = b_i^2 rho_i - b_i dt rho0 Dop_i u_i + s_i,
+//   p = c^2 sum_i rho_i,                    y = S p
+// with a = pml_sg, b = pml, beta_i = dt / rho0_sg_i.  The operator tables are
+// Hermitian, so Gop_i^T = -Dop_i and Dop_i^T = -Gop_i: the reverse step is the
+// forward one with gradient and divergence swapped.  One reverse step:
+//   k_adj_scatter   pbar += S^T r_t
+//   k_adj_eos       rbar_i += c^2 pbar; cbar += 2 p pbar / c; R = sum_i fac_i rbar_i;
+//                   w_i = b_i dt rho0 rbar_i -> F_z;  rbar_i = b_i^2 rbar_i
+//   (x/y passes)    Gop_i on w_i
+//   k_adj_mom       ubar_i += Gop_i w_i; v_i = a_i beta_i ubar_i -> F_z; ubar_i = a_i^2 ubar_i
+//   (x/y passes)    Dop_i on v_i
+//   k_adj_sum       pbar = sum_i Dop_i v_i
+//   k_adj_srcgrad   cbar += -(scale / c) sig(t) [K R] at the source points
+#pragma once
+#include "kw_kernels.hpp"
+
+namespace kw {
+
+template <int N, int ND>
+__global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_eos(Geom g, Fields fl, AdjFields af, const Inst* __restrict__ inst,
+                                                         const float2* __restrict__ tw_g, int t) {
+  KW_ZPASS_PROLOGUE
+  const Inst& I = inst[b];
+  __shared__ float pmlz[N];
+  for (int i_ = tid; i_ < N; i_ += C::NT) pmlz[i_] = gp(I.pml[ND - 1])[i_];
+  __syncthreads();
+  KW_ROWS_CLAMPED
+  const int64_t off = b * g.ntot;
+  const Src& src = I.srcp;
+  const bool srcon = src.mode != SRC_NONE && t < src.len;
+  float ph[ND][2];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) pml_rows(I.pml, g, i, rA, ph[i]);
+  const auto pb = gp(af.pb + off);
+  const auto cb = gw(af.cb + off);
+  const auto pt = gp(af.pt + off);
+  float2 pbv[C::E], c2v[C::E], rs[C::E];
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const int64_t i0 = iA + m * C::TPL;
+    pbv[m] = make_float2(pb[i0], pb[i0 + N]);
+    c2v[m] = make_float2(ld(I.c0sq, i0, I.c0sq_s), ld(I.c0sq, i0 + N, I.c0sq_s));
+    rs[m] = make_float2(0.f, 0.f);
+    // p = c^2 sum rho  ->  dJ/dc += 2 c sum(rho) pbar = 2 p pbar / c
+    const float ga = 2.f * pt[i0] * pbv[m].x / sqrtf(c2v[m].x);
+    const float gb = 2.f * pt[i0 + N] * pbv[m].y / sqrtf(c2v[m].y);
+    if (inside) {
+      cb[i0] = cb[i0] + ga;
+      cb[i0 + N] = cb[i0 + N] + gb;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    const auto rb = gw(af.rb[i] + off);
+    float2 w[C::E];
+#pragma unroll
+    for (int m = 0; m < C::E; ++m) {
+      const float sz = (i == ND - 1) ? pmlz[j0 + m * C::TPL] : 1.f;
+      float wv[2], rn[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int64_t idx = iA + h * N + m * C::TPL;
+        const float s = sz * ph[i][h];
+        const float rbt = rb[idx] + (h ? c2v[m].y : c2v[m].x) * (h ? pbv[m].y : pbv[m].x);
+        if (h) rs[m].y += src.fac[i] * rbt; else rs[m].x += src.fac[i] * rbt;
+        wv[h] = s * I.dt * ld(I.rho0, idx, I.rho0_s) * rbt;
+        rn[h] = s * s * rbt;
+      }
+      if (inside) {
+        rb[iA + m * C::TPL] = rn[0];
+        rb[iA + N + m * C::TPL] = rn[1];
+      }
+      w[m] = make_float2(wv[0], wv[1]);
+    }
+    fwd_from_regs<N>(lds, w, fl.spec[i] + b * g.nspec, tw, row0, rows, g.nhp, tid);
+  }
+  if (srcon && inside) {
+    const auto R = gw(af.R + off);
+#pragma unroll
+    for (int m = 0; m < C::E; ++m) {
+      R[iA + m * C::TPL] = rs[m].x;
+      R[iA + N + m * C::TPL] = rs[m].y;
+    }
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_mom(Geom g, Fields fl, AdjFields af, const Inst* __restrict__ inst,
+                                                         const float2* __restrict__ tw_g) {
+  KW_ZPASS_PROLOGUE
+  const int ax = blockIdx.z;
+  const Inst& I = inst[b];
+  __shared__ float pmlz[N];  // read after the first transform barrier
+  const bool zax = ax == g.nd - 1;
+  if (zax)
+    for (int i_ = tid; i_ < N; i_ += C::NT) pmlz[i_] = gp(I.pmlsg[ax])[i_];
+  KW_ROWS_CLAMPED
+  float2* sp = fl.spec[ax] + b * g.nspec;
+  const auto ub = gw(af.ub[ax] + b * g.ntot);
+  float ph[2];
+  pml_rows(I.pmlsg, g, ax, rA, ph);
+  float2 uo[C::E], dr[C::E];
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const int64_t i0 = iA + m * C::TPL;
+    uo[m] = make_float2(ub[i0], ub[i0 + N]);
+    dr[m] = make_float2(ld(I.dtr0[ax], i0, I.dtr0_s), ld(I.dtr0[ax], i0 + N, I.dtr0_s));
+  }
+  float2 gv[C::E];
+  inv_to_regs<N>(lds, sp, tw, row0, rows, g.nhp, tid, gv);
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const float sz = zax ? pmlz[j0 + m * C::TPL] : 1.f;
+    float un[2], vv[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const float s = sz * ph[h];
+      const float u = (h ? uo[m].y : uo[m].x) + (h ? gv[m].y : gv[m].x);
+      vv[h] = s * (h ? dr[m].y : dr[m].x) * u;
+      un[h] = s * s * u;
+    }
+    if (inside) {
+      ub[iA + m * C::TPL] = un[0];
+      ub[iA + N + m * C::TPL] = un[1];
+    }
+    gv[m] = make_float2(vv[0], vv[1]);
+  }
+  fwd_from_regs<N>(lds, gv, sp, tw, row0, rows, g.nhp, tid);
+}
+
+template <int N, int ND>
+__global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_sum(Geom g, Fields fl, AdjFields af, const Inst* __restrict__ inst,
+                                                         const float2* __restrict__ tw_g) {
+  KW_ZPASS_PROLOGUE
+  (void)inst;
+  KW_ROWS_CLAMPED
+  float2 acc[C::E];
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) acc[m] = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    float2 d[C::E];
+    inv_to_regs<N>(lds, fl.spec[i] + b * g.nspec, tw, row0, rows, g.nhp, tid, d);
+#pragma unroll
+    for (int m = 0; m < C::E; ++m) {
+      acc[m].x += d[m].x;
+      acc[m].y += d[m].y;
+    }
+  }
+  if (inside) {
+    const auto pb = gw(af.pb + b * g.ntot);
+#pragma unroll
+    for (int m = 0; m < C::E; ++m) {
+      pb[iA + m * C::TPL] = acc[m].x;
+      pb[iA + N + m * C::TPL] = acc[m].y;
+    }
+  }
+}
+
+}  // namespace kw

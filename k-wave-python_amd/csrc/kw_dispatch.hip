@@ -25,6 +25,9 @@ KW_DECL4(hipError_t, launch_momentum, const Geom&, int, const Fields&, const Ins
 KW_DECL4(hipError_t, launch_mass, const Geom&, int, const Fields&, const Inst*, const float2*, int, hipStream_t, bool)
 KW_DECL4(hipError_t, launch_eos, const Geom&, int, const Fields&, const Inst*, const float2*, int, hipStream_t)
 KW_DECL4(hipError_t, launch_col, const Geom&, int, const Fields&, const Ops&, const Inst*, const ColArgs&, int, hipStream_t)
+KW_DECL4(hipError_t, launch_adj_eos, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, int, hipStream_t)
+KW_DECL4(hipError_t, launch_adj_mom, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, hipStream_t)
+KW_DECL4(hipError_t, launch_adj_sum, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, hipStream_t)
 
 hipError_t launch_zfwd(const Geom& g, int nb, const float* f, int64_t fs, float2* spec, int64_t ss, const float2* tw, hipStream_t st) {
   KW_TRY4(launch_zfwd, g, nb, f, fs, spec, ss, tw, st);
@@ -49,6 +52,16 @@ hipError_t launch_eos(const Geom& g, int nb, const Fields& fl, const Inst* inst,
 }
 hipError_t launch_col(const Geom& g, int nb, const Fields& fl, const Ops& ops, const Inst* inst, const ColArgs& a, int nline, hipStream_t st) {
   KW_TRY4(launch_col, g, nb, fl, ops, inst, a, nline, st);
+}
+
+hipError_t launch_adj_eos(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, int t, hipStream_t st) {
+  KW_TRY4(launch_adj_eos, g, nb, fl, af, inst, tw, t, st);
+}
+hipError_t launch_adj_mom(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st) {
+  KW_TRY4(launch_adj_mom, g, nb, fl, af, inst, tw, st);
+}
+hipError_t launch_adj_sum(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st) {
+  KW_TRY4(launch_adj_sum, g, nb, fl, af, inst, tw, st);
 }
 
 }  // namespace kw

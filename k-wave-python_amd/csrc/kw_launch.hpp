@@ -19,4 +19,10 @@ hipError_t launch_gather(const Geom& g, int nb, int max_sens, const Fields& fl, 
 hipError_t launch_tick(const Fields& fl, const Inst* inst, int nb, hipStream_t st);
 hipError_t launch_fill(float* p, int64_t n, float v, hipStream_t st);
 hipError_t launch_setint(int* p, int n, int v, hipStream_t st);
+// reverse sweep (kw_adjoint.hpp)
+hipError_t launch_adj_scatter(const Geom& g, int nb, int max_sens, const AdjFields& af, const Inst* inst, int t, hipStream_t st);
+hipError_t launch_adj_srcgrad(const Geom& g, int nb, const AdjFields& af, const float* Z, const Inst* inst, int t, hipStream_t st);
+hipError_t launch_adj_eos(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, int t, hipStream_t st);
+hipError_t launch_adj_mom(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st);
+hipError_t launch_adj_sum(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st);
 }  // namespace kw

@@ -53,6 +53,62 @@ __global__ void k_setint(int* __restrict__ p, int n, int v) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// reverse sweep (kw_plan_adjoint_c; see kw_adjoint.hpp)
+// ---------------------------------------------------------------------------
+// pbar += S^T r_t: the transpose of k_gather for step t.  Binary masks hit
+// distinct voxels; interpolated points may share corners and add atomically.
+__global__ void k_adj_scatter(Geom g, AdjFields af, const Inst* __restrict__ inst, int t) {
+  const int b = blockIdx.y;
+  const Inst& I = inst[b];
+  const int j = t - I.rec_start;
+  if (I.sens_kind == 0 || j < 0 || j >= af.nrec) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= I.nsens) return;
+  const float r = af.adj[((int64_t)b * af.nrec + j) * af.nsens + s];
+  float* pb = af.pb + b * g.ntot;
+  if (I.ncorner == 1) {
+    pb[I.sidx[s]] += r;
+  } else {
+    for (int c = 0; c < I.ncorner; ++c) {
+      const int64_t k = (int64_t)s * I.ncorner + c;
+      atomicAdd(pb + I.sidx[k], (I.sens_kind == 2 ? I.sw[k] : 1.f) * r);
+    }
+  }
+}
+
+// Source-scale term: the injected value scale * sig(t) with scale = 2 dt / (D c d)
+// (kspace_solver.py:619-624) depends on c at the source point, so
+// cbar += -(scale / c) sig(t) Z with Z = K R (additive) or R (uncorrected).
+// One thread per grid row walks the row's source points (CSR).
+__global__ void k_adj_srcgrad(Geom g, AdjFields af, const float* __restrict__ Zk, const Inst* __restrict__ inst, int t) {
+  const int b = blockIdx.y;
+  const Inst& I = inst[b];
+  const Src& s = I.srcp;
+  if (s.mode == SRC_NONE || s.mode == SRC_DIRICHLET || t >= s.len) return;
+  const float* Z = (s.mode == SRC_NOCORR ? af.R : Zk) + b * g.ntot;
+  float* cb = af.cb + b * g.ntot;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < g.rows; r += gridDim.x * blockDim.x) {
+    for (int e = s.row_ptr[r]; e < s.row_ptr[r + 1]; ++e) {
+      const int64_t idx = (int64_t)r * g.nz + s.col[e];
+      const float c = sqrtf(I.c0sq[idx * I.c0sq_s]);
+      const float sv = s.sig[(int64_t)s.sigrow[e] * s.len + t];
+      cb[idx] += -(s.scale[e] / c) * sv * Z[idx];
+    }
+  }
+}
+
+hipError_t launch_adj_scatter(const Geom& g, int nb, int max_sens, const AdjFields& af, const Inst* inst, int t, hipStream_t st) {
+  if (max_sens <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adj_scatter, dim3((max_sens + 255) / 256, nb), dim3(256), 0, st, g, af, inst, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_srcgrad(const Geom& g, int nb, const AdjFields& af, const float* Z, const Inst* inst, int t, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_srcgrad, dim3((g.rows + 255) / 256, nb), dim3(256), 0, st, g, af, Z, inst, t);
+  return hipGetLastError();
+}
+
 bool size_supported(int n) { return size_supported_impl(n); }
 
 hipError_t launch_gather(const Geom& g, int nb, int max_sens, const Fields& fl, const Inst* inst, hipStream_t st) {

@@ -112,4 +112,16 @@ struct Fields {
   int* tdev;        // per-instance step counter
 };
 
+// State of the reverse-time sweep of kw_plan_adjoint_c (all [nb][ntot] unless noted).
+struct AdjFields {
+  float* pb;         // adjoint of p
+  float* ub[3];      // adjoint of u_i
+  float* rb[3];      // adjoint of rho_i
+  float* cb;         // accumulated dJ/dc0
+  float* R;          // sum_i rb_i * fac_i on source-active steps
+  const float* pt;   // forward p of the step being reversed
+  const float* adj;  // dJ/d(recorded p), [nb][nrec][nsens]
+  int nrec, nsens;
+};
+
 }  // namespace kw

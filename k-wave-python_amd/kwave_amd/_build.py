@@ -17,7 +17,7 @@ INCLUDE = PKG.parent.parent / "include"
 OUT_DIR = PKG / "_lib"
 LIB = OUT_DIR / "libkwgpu.so"
 # launcher files compiled once per line-length family (-DKW_GROUP=g), heaviest first
-GROUPED = ["kw_zpass_mass.hip", "kw_col_pass.hip", "kw_zpass_mom.hip", "kw_zpass_eos.hip", "kw_zpass_io.hip"]
+GROUPED = ["kw_zpass_mass.hip", "kw_col_pass.hip", "kw_zpass_mom.hip", "kw_zpass_eos.hip", "kw_zpass_io.hip", "kw_adjoint.hip"]
 GROUPS = (3, 2, 1, 0)
 SOURCES = ["kw_dispatch.hip", "kw_misc.hip", "kw_abi.hip"]
 

@@ -1,0 +1,94 @@
+"""Gradient of recorded pressure traces with respect to sound speed (reverse-time adjoint on the GPU).
+
+The reference solver (kwave/solvers/kspace_solver.py) is forward-only.  This
+module differentiates its step (``Simulation.step``, :691-747) for linear
+lossless media with additive pressure sources and binary or Cartesian
+sensors: ``sound_speed_gradient`` returns dJ/dc0 for a given dJ/d(traces), and
+``sensor_traces`` wraps forward + adjoint as a ``torch.autograd.Function`` so a
+misfit written in torch can be minimised over the sound-speed map.
+
+The forward state is checkpointed every ``checkpoint`` steps and replayed one
+segment at a time (memory: (2D+1) grids per checkpoint plus ``checkpoint``
+pressure grids).  c_ref = max(c0) (kspace_solver.py:231), dt, the PML and kappa
+are held fixed: the gradient is exact for perturbations that keep max(c0).
+"""
+from __future__ import annotations
+
+import copy
+
+import numpy as np
+
+from . import _abi
+from .solvers.plan import DevicePlan
+from .solvers.setup import build_instance
+
+
+def _plan(kgrid, medium, source, sensor, c, pml_size, pml_alpha, **kw):
+    med = copy.copy(medium)
+    if c is not None:
+        med.sound_speed = np.asarray(c, dtype=np.float64)
+    st = build_instance(kgrid, med, copy.deepcopy(source), sensor, pml_size=pml_size, pml_alpha=pml_alpha, **kw)
+    if "p" not in st.record:
+        raise ValueError("the sensor must record 'p'")
+    return DevicePlan([st]), st
+
+
+def forward_traces(kgrid, medium, source, sensor, *, c=None, pml_size=None, pml_alpha=None, **kw):
+    """Recorded p of one simulation, (n_sensor, n_recorded_steps) float32."""
+    plan, st = _plan(kgrid, medium, source, sensor, c, pml_size, pml_alpha, **kw)
+    try:
+        plan.run(st.Nt - plan.step_index(0))
+        return plan.recorded(0, _abi.F_REC_P).copy()
+    finally:
+        plan.close()
+
+
+def sound_speed_gradient(kgrid, medium, source, sensor, adj_src, *, c=None, pml_size=None, pml_alpha=None,
+                         checkpoint=50, **kw):
+    """(dJ/dc0 as a grid, recorded p) for dJ/d(recorded p) = ``adj_src`` (n_sensor, n_rec)."""
+    plan, st = _plan(kgrid, medium, source, sensor, c, pml_size, pml_alpha, **kw)
+    try:
+        grad = plan.adjoint_c([adj_src], checkpoint=checkpoint)[0]
+        return grad, plan.recorded(0, _abi.F_REC_P).copy()
+    finally:
+        plan.close()
+
+
+def _autograd():
+    import torch
+
+    class SensorTraces(torch.autograd.Function):
+        """c0 map -> recorded p traces; backward runs the reverse-time adjoint."""
+
+        @staticmethod
+        def forward(ctx, c, problem):
+            c_np = c.detach().to("cpu", torch.float64).numpy()
+            kg, med, src, sen, kw = problem
+            kw = {k: v for k, v in kw.items() if k != "checkpoint"}
+            traces = forward_traces(kg, med, src, sen, c=c_np, **kw)
+            ctx.problem, ctx.c = problem, c_np
+            ctx.dtype, ctx.device = c.dtype, c.device
+            return torch.from_numpy(traces).to(device=c.device, dtype=torch.float32)
+
+        @staticmethod
+        def backward(ctx, g):
+            kg, med, src, sen, kw = ctx.problem
+            adj = g.detach().to("cpu", torch.float32).numpy()
+            grad, _ = sound_speed_gradient(kg, med, src, sen, adj, c=ctx.c, **kw)
+            return torch.from_numpy(grad).to(device=ctx.device, dtype=ctx.dtype), None
+
+    return SensorTraces
+
+
+_SensorTraces = None
+
+
+def sensor_traces(c, kgrid, medium, source, sensor, **kw):
+    """Differentiable recorded p (torch tensor, (n_sensor, n_rec)) as a function of the c0 tensor ``c``.
+
+    Keywords go to the set-up (``pml_size``, ``pml_alpha``, ``checkpoint`` ...).
+    """
+    global _SensorTraces
+    if _SensorTraces is None:
+        _SensorTraces = _autograd()
+    return _SensorTraces.apply(c, (kgrid, medium, source, sensor, kw))

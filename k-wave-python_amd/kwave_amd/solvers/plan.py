@@ -153,6 +153,25 @@ class DevicePlan:
     def field(self, b, which):
         return self.read(b, which).reshape(self.shape)
 
+    def adjoint_c(self, adj, checkpoint=50):
+        """dJ/dc0 per instance by the reverse-time adjoint (kw_plan_adjoint_c).
+
+        adj[b] is dJ/d(recorded p) of instance b, shaped like recorded(b, F_REC_P)
+        (n_sensor, n_recorded_steps).  Returns a list of grid-shaped float64
+        gradients.  Re-runs the forward problem from t = 0; call reset() before
+        stepping the plan again.
+        """
+        rows = []
+        for b, s in enumerate(self.setups):
+            a = np.asarray(adj[b], dtype=np.float32)
+            if a.shape != (s.n_sensor, s.Nt - s.rec_start):
+                raise ValueError(f"adjoint source of instance {b} has shape {a.shape}, expected {(s.n_sensor, s.Nt - s.rec_start)}")
+            rows.append(a.T)
+        a = np.ascontiguousarray(np.stack(rows), dtype=np.float32)
+        out = np.empty((len(self.setups),) + tuple(self.shape), dtype=np.float32)
+        check(_abi.lib().kw_plan_adjoint_c(self._h, fptr(a), int(checkpoint), fptr(out)))
+        return [out[b].astype(np.float64) for b in range(len(self.setups))]
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _abi.lib().kw_plan_destroy(self._h)

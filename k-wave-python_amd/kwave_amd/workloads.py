@@ -15,6 +15,8 @@ frozen here so results are comparable across runs.
                   planar sensor recording p and p_max, 1500 steps.
 * ``scan_line_2d`` / ``scan_sweep_2d``  uneven 2-D linear-array sweep
                   (96 instances of 256^2 .. 640^2, 1200-2600 steps).
+* ``surrogate_2d`` 128^2 sound-speed map -> 32 sensor traces of 600 steps: the
+                  samples of the learned surrogate and the adjoint problem.
 The grid edge ``n`` and step count ``nt`` arguments shrink the problems for
 parity tests.
 """
@@ -199,3 +201,59 @@ def scan_line_2d(seed, shape=None, nt=None, pml=20):
 def scan_sweep_2d(seed=4, count=96):
     """The 96-instance uneven sweep: instance i uses seed 1000 * seed + i."""
     return [scan_line_2d(1000 * seed + i) for i in range(count)]
+
+
+SURROGATE_C_RANGE = (1400.0, 1650.0)
+
+
+def surrogate_c_map(seed, n=128):
+    """Sound-speed map of one surrogate sample.
+
+    Gaussian random field (white noise, periodic Gaussian filter with a
+    correlation length of 10 cells) mapped affinely onto [1400, 1650] m/s, plus
+    0-3 ellipses (semi-axes 6-20 cells, random angle) at a uniform value in the
+    same range, then a 1-cell Gaussian smoothing.
+    """
+    rng = np.random.default_rng(seed)
+    lo, hi = SURROGATE_C_RANGE
+    c = _affine(_gaussian_filter_periodic(rng.standard_normal((n, n)), 10.0 * n / 128), lo, hi)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    s = n / 128
+    for _ in range(int(rng.integers(0, 4))):
+        cx, cy = rng.uniform(30 * s, n - 20 * s, size=2)
+        a, b = rng.uniform(6 * s, 20 * s, size=2)
+        th = rng.uniform(0, np.pi)
+        x, y = ii - cx, jj - cy
+        u = x * np.cos(th) + y * np.sin(th)
+        v = -x * np.sin(th) + y * np.cos(th)
+        c[(u / a) ** 2 + (v / b) ** 2 <= 1] = rng.uniform(lo, hi)
+    from scipy.ndimage import gaussian_filter
+
+    return np.clip(gaussian_filter(c, 1.0, mode="reflect"), lo, hi)
+
+
+def surrogate_2d(seed, n=128, nt=600, c=None):
+    """One surrogate sample: c map (seeded, or given) and the fixed acquisition.
+
+    128^2, dx = 0.1 mm, rho = 1000, lossless.  dt = 0.3 dx / 1650 (the top of the
+    c range, so dt and Nt do not depend on the sample).  Source: 32-cell line at
+    x = 24, y in [48, 80), additive, tone_burst(1/dt, 1.5 MHz, 3 cycles) x 1e5.
+    Sensor: the same 32 cells, recording p.  PML 20, alpha 2.
+    """
+    s = n / 128
+    dx = 1e-4
+    if c is None:
+        c = surrogate_c_map(seed, n)
+    kgrid = kWaveGrid(Vector([n, n]), Vector([dx, dx]))
+    dt = 0.3 * dx / SURROGATE_C_RANGE[1]
+    kgrid.setTime(int(nt), dt)
+    medium = kWaveMedium(sound_speed=np.asarray(c, dtype=np.float64), density=1000.0)
+    mask = np.zeros((n, n), dtype=bool)
+    mask[int(round(24 * s)), int(round(48 * s)):int(round(80 * s))] = True
+    source = kSource()
+    source.p_mask = mask
+    source.p = 1e5 * tone_burst(1 / dt, 1.5e6, 3)
+    sensor = kSensor(mask.copy())
+    sensor.record = ["p"]
+    pml = max(2, int(round(20 * s)))
+    return kgrid, medium, source, sensor, (pml, pml), (2.0, 2.0)

@@ -1,0 +1,108 @@
+"""Pin the FP64 adjoint oracle (oracle/kspace_adjoint.py) with central finite differences.
+
+The functional is J(c) = 1/2 sum (y(c) - d)^2 over the recorded sensor traces of
+OracleSimulation (the FP64 restatement of kwave/solvers/kspace_solver.py:691-747).
+Cells are drawn away from max(c0), so that c_ref = max(c0) (kspace_solver.py:231),
+which the adjoint holds fixed, does not move; two of them are source points, which
+carry the source-scale term 2 dt / (D c d_i) (kspace_solver.py:619-624).
+"""
+import copy
+
+import numpy as np
+import pytest
+
+
+def adjoint_case(shape=(32, 40), nt=90, seed=7, mode="additive"):
+    from kwave_amd.data import Vector
+    from kwave_amd.kgrid import kWaveGrid
+    from kwave_amd.kmedium import kWaveMedium
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.ksource import kSource
+    from kwave_amd.utils.signals import tone_burst
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(seed)
+    dx = 1e-4
+    kg = kWaveGrid(Vector(list(shape)), Vector([dx] * len(shape)))
+    kg.setTime(nt, 0.3 * dx / 1700.0)
+    c = 1500.0 + 150.0 * gaussian_filter(rng.random(shape), 2)
+    rho = 1000.0 + 100.0 * gaussian_filter(rng.random(shape), 2)
+    med = kWaveMedium(sound_speed=c, density=rho)
+    src = kSource()
+    mask = np.zeros(shape, dtype=bool)
+    mask[(8,) + tuple(slice(s // 2 - 3, s // 2 + 3) for s in shape[1:])] = True
+    src.p_mask = mask
+    src.p = 1e4 * tone_burst(1.0 / kg.dt, 2.5e6, 2)
+    src.p_mode = mode
+    smask = np.zeros(shape, dtype=bool)
+    smask[(shape[0] - 9,) + tuple(slice(s // 4, 3 * s // 4, 2) for s in shape[1:])] = True
+    sen = kSensor(smask)
+    sen.record = ["p"]
+    return kg, med, src, sen, tuple(4 for _ in shape), tuple(2.0 for _ in shape)
+
+
+def _oracle(case, c):
+    from oracle.kspace_ref import OracleSimulation
+
+    kg, med, src, sen, pml, alpha = case
+    m = copy.deepcopy(med)
+    m.sound_speed = c
+    return OracleSimulation(kg, m, copy.deepcopy(src), sen, pml_size=pml, pml_alpha=alpha)
+
+
+def _cells(case, c, n_rand=6, seed=3):
+    rng = np.random.default_rng(seed)
+    flat = c.ravel()
+    ok = np.where(flat < flat.max() - 1.0)[0]
+    cells = list(rng.choice(ok, size=n_rand, replace=False))
+    src_pts = np.where(np.asarray(case[2].p_mask).ravel())[0]
+    cells += list(src_pts[[0, src_pts.size // 2]])
+    return cells
+
+
+@pytest.mark.parametrize("mode", ["additive", "additive-no-correction"])
+def test_oracle_adjoint_matches_finite_differences(mode):
+    from oracle.kspace_adjoint import sound_speed_gradient
+
+    case = adjoint_case(mode=mode)
+    c = np.asarray(case[1].sound_speed, dtype=np.float64)
+    rec0 = _oracle(case, c).run()["p"]
+    d = rec0 * 0.7 + 0.05 * np.abs(rec0).max() * np.random.default_rng(5).standard_normal(rec0.shape)
+    grad, rec = sound_speed_gradient(_oracle(case, c), rec0 - d)
+    assert np.allclose(rec, rec0, rtol=0, atol=0)
+
+    def J(cc):
+        return 0.5 * float(np.sum((_oracle(case, cc).run()["p"] - d) ** 2))
+
+    h = 1e-2
+    for cell in _cells(case, c):
+        cp, cm = c.copy(), c.copy()
+        cp.ravel()[cell] += h
+        cm.ravel()[cell] -= h
+        fd = (J(cp) - J(cm)) / (2 * h)
+        g = grad.ravel()[cell]
+        assert abs(g - fd) <= 1e-6 * max(abs(fd), 1e-12 * np.abs(grad).max()) + 1e-9 * np.abs(grad).max(), (cell, g, fd)
+
+
+def test_oracle_adjoint_planted_sign_error_is_caught():
+    """A dropped source-scale term (a plausible slip) fails the same check at a source point."""
+    from oracle import kspace_adjoint
+
+    case = adjoint_case()
+    c = np.asarray(case[1].sound_speed, dtype=np.float64)
+    rec0 = _oracle(case, c).run()["p"]
+    d = 0.5 * rec0
+    orig = kspace_adjoint._source_points
+    try:
+        kspace_adjoint._source_points = lambda sim: None
+        grad, _ = kspace_adjoint.sound_speed_gradient(_oracle(case, c), rec0 - d)
+    finally:
+        kspace_adjoint._source_points = orig
+    cell = int(np.where(np.asarray(case[2].p_mask).ravel())[0][0])
+    h = 1e-2
+    cp, cm = c.copy(), c.copy()
+    cp.ravel()[cell] += h
+    cm.ravel()[cell] -= h
+    J = lambda cc: 0.5 * float(np.sum((_oracle(case, cc).run()["p"] - d) ** 2))  # noqa: E731
+    fd = (J(cp) - J(cm)) / (2 * h)
+    assert abs(grad.ravel()[cell] - fd) > 1e-3 * abs(fd)
