@@ -12,6 +12,13 @@ state is re-initialised, then exactly K steps are timed between two device
 synchronisations, with HIP events bracketing every kernel launch on the plan
 stream (per-kernel device time, used for the roofline of the dominant kernel).
 
+Further legs (skippable): the two 2-D workloads (128^2 IVP, 512^2 absorbing),
+the 96-instance uneven 2-D sweep batched vs looped, the learned surrogate
+(trained on 4096 solver samples of the 128^2 sound-speed -> traces problem,
+held-out error and speed against the batched solver) and the reverse-time
+adjoint (gradient time against forward time), and the FP64 CPU oracle as the
+CPU baseline.
+
 Prints ONE JSON line on stdout.  Everything else goes to stderr.
 """
 import argparse
@@ -150,6 +157,42 @@ def small_workloads():
     return out
 
 
+def surrogate_leg():
+    """Learned surrogate of the 128^2 problem: train on solver data, held-out error, speed vs the solver."""
+    from kwave_amd import surrogate
+
+    res, _ = surrogate.benchmark(n_train=4096, n_test=512, epochs=80, log=log)
+    return res
+
+
+def adjoint_leg(checkpoint=50):
+    """Reverse-time gradient dJ/dc0 on the 128^2, 600-step problem vs one forward run (same plan)."""
+    from kwave_amd import _abi, workloads
+    from kwave_amd.solvers.plan import DevicePlan
+    from kwave_amd.solvers.setup import build_instance
+
+    kg, med, src, sen, pml, alpha = workloads.surrogate_2d(seed=5_000_321)
+    st = build_instance(kg, med, src, sen, pml_size=pml, pml_alpha=alpha)
+    plan = DevicePlan([st])
+    try:
+        plan.run(st.Nt)
+        plan.reset()
+        plan.run(0)
+        t0 = time.perf_counter()
+        plan.run(st.Nt)
+        t_fwd = time.perf_counter() - t0
+        adj = 0.3 * plan.recorded(0, _abi.F_REC_P)
+        plan.adjoint_c([adj], checkpoint=checkpoint)
+        t0 = time.perf_counter()
+        plan.adjoint_c([adj], checkpoint=checkpoint)
+        t_adj = time.perf_counter() - t0
+    finally:
+        plan.close()
+    log(f"adjoint: forward {1e3 * t_fwd:.1f} ms, gradient {1e3 * t_adj:.1f} ms ({t_adj / t_fwd:.1f}x)")
+    return {"grid": list(st.shape), "nt": st.Nt, "checkpoint": checkpoint, "forward_ms": 1e3 * t_fwd,
+            "gradient_ms": 1e3 * t_adj, "gradient_over_forward": t_adj / t_fwd}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,12 +201,17 @@ def main():
     ap.add_argument("--n", type=int, default=512, help="grid edge (512 is the benchmark workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-scan", action="store_true", help="skip the batched 96-instance 2-D sweep")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate and adjoint legs")
     args = ap.parse_args()
     if args.gpus != 1:
         log("this benchmark runs on one GPU only: the 512^3 workload is a single simulation on one MI355X; "
             "multi-GPU sharding (slab decomposition with all-to-all transposes) is a design note, not a build")
         sys.exit(2)
 
+    if not args.no_surrogate:
+        import torch  # the surrogate leg uses PyTorch: initialise its GPU context before the solver library
+
+        torch.cuda.init()
     from kwave_amd import workloads
     from kwave_amd.solvers.plan import DevicePlan
     from kwave_amd.solvers.setup import build_instance
@@ -207,7 +255,7 @@ def main():
     p_rec = plan.read(0, _abi.F_REC_P)
     finite = bool(abs(p_rec).max() < 1e12)
     out = {
-        "metric": "voxel-steps/s, %HBM peak (512^3)",
+        "metric": "voxel-steps/s, %HBM peak (512\u00b3); batched sims/s; surrogate speedup @2% err",
         "value": rate,
         "unit": "voxel-steps/s",
         "n_gpus": 1,
@@ -242,6 +290,13 @@ def main():
         except Exception as e:
             log(f"scan benchmark failed: {e!r}")
             out["batched_scan"] = None
+    if not args.no_surrogate:
+        for key, leg in (("surrogate", surrogate_leg), ("adjoint", adjoint_leg)):
+            try:
+                out[key] = leg()
+            except Exception as e:
+                log(f"{key} leg failed: {e!r}")
+                out[key] = None
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline()

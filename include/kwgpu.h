@@ -162,6 +162,28 @@ int kw_plan_destroy(kw_plan* plan);
  * Simulation.step (kspace_solver.py:691-747) and source scaling (:619-624). */
 int kw_plan_adjoint_c(kw_plan* plan, const float* adj, int ckpt, float* grad_c);
 
+/* Mode-batched GEMM of the Fourier-neural-operator surrogate, BF16 inputs with
+ * FP32 accumulation on the matrix cores:  C[m] = A[m] * Bt[m]^T for m < nmodes,
+ * A: [nmodes][R][K] bf16, Bt: [nmodes][N][K] bf16, C: [nmodes][R][N] fp32, all
+ * device pointers (row-major, contiguous, bf16 operands 8-byte aligned),
+ * R and N multiples of 32, K a multiple of 8; enqueued on `stream` (a
+ * hipStream_t, NULL for the default stream) without waiting.  The spectral
+ * layers' forward and both gradients are such GEMMs over real-stacked complex
+ * modes.  The reference has no learned surrogate; no counterpart. */
+int kw_mode_gemm_bf16(int nmodes, int R, int N, int K, const void* A, const void* Bt, float* C, void* stream);
+
+/* The surrogate's spectral contraction with the mode gather and scatter fused:
+ * for every retained mode (kx rows 0..md-1 and NX-md..NX-1, ky < md) and batch
+ * row b < B, Y[b][:, kx, ky] = X[b][:, kx, ky] W[mode] with complex channels
+ * interleaved as (re, im) pairs, i.e. the real GEMM [B x 2C] x [2C x 2C] whose
+ * bf16 transposed weights are Bt [2 md md][2C][2C] (row n = output index).  X
+ * and Y are complex64 device arrays [B][C][NX][NY] given by their element
+ * strides {batch, channel, kx, ky}; Y is written only at the retained modes.
+ * 2C must be a multiple of 32.  Enqueued on `stream`.  No counterpart in the
+ * reference (it has no surrogate). */
+int kw_mode_contract_bf16(int B, int C, int md, int NX, int NY, const void* X, const int64_t xstride[4], const void* Bt, void* Y,
+                          const int64_t ystride[4], void* stream);
+
 /* Spectral operator f -> real(ifftn(op * fftn(f))) on the device, the
  * reference's Simulation._diff (kspace_solver.py:781-796), for a single field.
  * kind: 0 identity, 1 gradient (op_grad), 2 divergence (op_div),

@@ -967,6 +967,32 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   return KW_OK;
 }
 
+int kw_mode_gemm_bf16(int nmodes, int R, int N, int K, const void* A, const void* Bt, float* C, void* stream) {
+  if (!A || !Bt || !C || nmodes < 1 || R < 32 || N < 32 || K < 8) return fail(KW_EINVAL, "mode GEMM: NULL pointer or empty shape");
+  if (R % 32 || N % 32 || K % 8) return fail(KW_EINVAL, "mode GEMM: R and N must be multiples of 32, K of 8");
+  if (((uintptr_t)A | (uintptr_t)Bt) & 7) return fail(KW_EINVAL, "mode GEMM: bf16 operands must be 8-byte aligned");
+  if ((int64_t)nmodes > 65535) return fail(KW_EINVAL, "mode GEMM: at most 65535 modes per call");
+  hipError_t e = launch_mode_gemm(nmodes, R, N, K, static_cast<const uint16_t*>(A), static_cast<const uint16_t*>(Bt), C,
+                                  static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(KW_EHIP, std::string("mode GEMM launch: ") + hipGetErrorString(e));
+  return KW_OK;
+}
+
+int kw_mode_contract_bf16(int B, int C, int md, int NX, int NY, const void* X, const int64_t xstride[4], const void* Bt, void* Y,
+                          const int64_t ystride[4], void* stream) {
+  if (!X || !Bt || !Y || !xstride || !ystride || B < 1 || md < 1) return fail(KW_EINVAL, "mode contraction: NULL pointer or empty shape");
+  if ((2 * C) % 32) return fail(KW_EINVAL, "mode contraction: 2 C must be a multiple of 32");
+  if (md > NY || 2 * md > NX) return fail(KW_EINVAL, "mode contraction: retained modes exceed the spectrum");
+  if ((uintptr_t)Bt & 7) return fail(KW_EINVAL, "mode contraction: bf16 weights must be 8-byte aligned");
+  if (2 * (int64_t)md * md > 65535) return fail(KW_EINVAL, "mode contraction: too many modes");
+  for (int i = 0; i < 4; ++i)
+    if (xstride[i] < 1 || ystride[i] < 1) return fail(KW_EINVAL, "mode contraction: strides must be positive");
+  hipError_t e = launch_mode_contract(B, 2 * C, md, NX, static_cast<const float2*>(X), xstride, static_cast<const uint16_t*>(Bt),
+                                      static_cast<float2*>(Y), ystride, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(KW_EHIP, std::string("mode contraction launch: ") + hipGetErrorString(e));
+  return KW_OK;
+}
+
 int kw_plan_destroy(kw_plan* P) {
   if (!P) return KW_OK;
   (void)hipSetDevice(P->dev);

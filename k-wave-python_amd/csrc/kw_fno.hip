@@ -1,0 +1,103 @@
+// Spectral-mode contraction of the Fourier neural operator surrogate (BF16 MFMA).
+//
+// An FNO layer multiplies the retained Fourier modes of its input channels by a
+// complex weight matrix per mode: Y[m] = X[m] W[m] (batch x Cin) (Cin x Cout).
+// With real stacking, X2 = [Re X, Im X] and W2 = [[Re W, Im W], [-Im W, Re W]],
+// forward, input gradient and weight gradient are all real mode-batched GEMMs
+//   C[m] = A[m] Bt[m]^T      A: [M][R][K], Bt: [M][N][K] (bf16), C: [M][R][N] (fp32)
+// (forward A = X2, Bt = W2^T; input gradient A = dY2, Bt = W2; weight gradient
+// A = X2^T, Bt = dY2^T).  One wavefront owns a 32 x 32 tile of C and consumes K
+// eight at a time with v_mfma_f32_32x32x8_bf16 (FP32 accumulation).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kw_launch.hpp"
+
+namespace kw {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Operand layout of v_mfma_f32_32x32x8_bf16 for lane l of the wavefront: the A
+// operand holds row l % 32, k = 4 (l / 32) + 0..3; the B operand column l % 32,
+// same k; result register r holds C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+__global__ __launch_bounds__(64) void k_mode_gemm(int R, int N, int K, const uint16_t* __restrict__ A,
+                                                  const uint16_t* __restrict__ Bt, float* __restrict__ C) {
+  const int m = blockIdx.z;
+  const int r0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+  const int l = threadIdx.x;
+  const int i = l & 31, kh = (l >> 5) * 4;
+  const uint16_t* a = A + ((int64_t)m * R + r0 + i) * K + kh;
+  const uint16_t* b = Bt + ((int64_t)m * N + n0 + i) * K + kh;
+  f32x16 acc = {};
+  for (int k = 0; k < K; k += 8) {
+    const bf16x4 av = *reinterpret_cast<const bf16x4*>(a + k);
+    const bf16x4 bv = *reinterpret_cast<const bf16x4*>(b + k);
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+  }
+  float* c = C + ((int64_t)m * R + r0) * N + n0 + i;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) c[(int64_t)(8 * (r >> 2) + 4 * (l >> 5) + (r & 3)) * N] = acc[r];
+}
+
+// float -> bfloat16 bits, round to nearest even (finite inputs)
+__device__ __forceinline__ short bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// The same contraction with the mode gather and scatter fused in: the A operand
+// is read straight from a complex spectrum X[b][c][kx][ky] at the retained modes (kx rows 0..md-1 and
+// NX-md..NX-1, ky < md; mode m = kx_index * md + ky), rounded to bf16 in
+// registers, and the result is written straight into the complex spectrum Y at
+// the same modes.  Strides (in complex elements) are free on both sides.
+// Real and imaginary parts are interleaved per channel
+// (k = 2 channel + part), so K = N = 2 C.  Batch rows >= B read zero and store
+// nothing, so the batch needs no padding.
+__global__ __launch_bounds__(64) void k_mode_contract(int B, int K, int md, int NX, const float2* __restrict__ X, int64_t xsb,
+                                                      int64_t xsc, int64_t xsx, int64_t xsy, const uint16_t* __restrict__ Bt,
+                                                      float* __restrict__ Y, int64_t ysb, int64_t ysc, int64_t ysx, int64_t ysy) {
+  const int m = blockIdx.z;
+  const int r0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+  const int l = threadIdx.x;
+  const int i = l & 31, kh = (l >> 5) * 4;
+  const int kxi = m / md, ky = m % md;
+  const int row = kxi < md ? kxi : NX - 2 * md + kxi;
+  const int r = r0 + i;
+  const bool rv = r < B;
+  const float2* xa = X + (int64_t)(rv ? r : 0) * xsb + (int64_t)row * xsx + (int64_t)ky * xsy;
+  const uint16_t* b = Bt + ((int64_t)m * K + n0 + i) * K + kh;
+  f32x16 acc = {};
+  for (int k = 0; k < K; k += 8) {
+    const int ch = (k + kh) >> 1;  // channels ch, ch + 1
+    bf16x4 av = {0, 0, 0, 0};
+    if (rv) {
+      const float2 x0 = xa[(int64_t)ch * xsc], x1 = xa[(int64_t)(ch + 1) * xsc];
+      av = bf16x4{bf16_rne(x0.x), bf16_rne(x0.y), bf16_rne(x1.x), bf16_rne(x1.y)};
+    }
+    const bf16x4 bv = *reinterpret_cast<const bf16x4*>(b + k);
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+  }
+  const int col = n0 + i, o = col >> 1, part = col & 1;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int rr = r0 + 8 * (q >> 2) + 4 * (l >> 5) + (q & 3);
+    if (rr < B) Y[2 * ((int64_t)rr * ysb + (int64_t)o * ysc + (int64_t)row * ysx + (int64_t)ky * ysy) + part] = acc[q];
+  }
+}
+
+hipError_t launch_mode_contract(int B, int K, int md, int NX, const float2* X, const int64_t xs[4], const uint16_t* Bt, float2* Y,
+                                const int64_t ys[4], hipStream_t st) {
+  hipLaunchKernelGGL(k_mode_contract, dim3(K / 32, (B + 31) / 32, 2 * md * md), dim3(64), 0, st, B, K, md, NX, X, xs[0], xs[1],
+                     xs[2], xs[3], Bt, reinterpret_cast<float*>(Y), ys[0], ys[1], ys[2], ys[3]);
+  return hipGetLastError();
+}
+
+hipError_t launch_mode_gemm(int M, int R, int N, int K, const uint16_t* A, const uint16_t* Bt, float* C, hipStream_t st) {
+  hipLaunchKernelGGL(k_mode_gemm, dim3(N / 32, R / 32, M), dim3(64), 0, st, R, N, K, A, Bt, C);
+  return hipGetLastError();
+}
+
+}  // namespace kw

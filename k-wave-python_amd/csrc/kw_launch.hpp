@@ -19,6 +19,10 @@ hipError_t launch_gather(const Geom& g, int nb, int max_sens, const Fields& fl, 
 hipError_t launch_tick(const Fields& fl, const Inst* inst, int nb, hipStream_t st);
 hipError_t launch_fill(float* p, int64_t n, float v, hipStream_t st);
 hipError_t launch_setint(int* p, int n, int v, hipStream_t st);
+// surrogate (kw_fno.hip)
+hipError_t launch_mode_gemm(int M, int R, int N, int K, const uint16_t* A, const uint16_t* Bt, float* C, hipStream_t st);
+hipError_t launch_mode_contract(int B, int K, int md, int NX, const float2* X, const int64_t xs[4], const uint16_t* Bt, float2* Y,
+                                const int64_t ys[4], hipStream_t st);
 // reverse sweep (kw_adjoint.hpp)
 hipError_t launch_adj_scatter(const Geom& g, int nb, int max_sens, const AdjFields& af, const Inst* inst, int t, hipStream_t st);
 hipError_t launch_adj_srcgrad(const Geom& g, int nb, const AdjFields& af, const float* Z, const Inst* inst, int t, hipStream_t st);

@@ -5,6 +5,7 @@ fallback: a missing library raises ImportError, so the HIP path is the only
 path.
 """
 import ctypes as C
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -24,7 +25,7 @@ EXPORTS = (
     "kw_plan_create", "kw_plan_set_instance", "kw_plan_set_field", "kw_plan_set_source",
     "kw_plan_set_sensor", "kw_plan_init", "kw_plan_run", "kw_plan_read", "kw_plan_field_size",
     "kw_plan_destroy", "kw_spectral_apply", "kw_plan_run_profiled", "kw_plan_launch", "kw_plan_sync", "kw_kernel_kind_count", "kw_kernel_kind_name",
-    "kw_plan_adjoint_c",
+    "kw_plan_adjoint_c", "kw_mode_gemm_bf16", "kw_mode_contract_bf16",
 )
 
 
@@ -45,6 +46,23 @@ class InstDesc(C.Structure):
 _lib = None
 
 
+def _torch_context_first():
+    """Initialise PyTorch's GPU context first when PyTorch is in the process.
+
+    The PyTorch wheel carries its own HIP runtime next to the system one this
+    library links; with both in one process, PyTorch has found no GPU when this
+    library initialised the device first, while the opposite order works.
+    """
+    torch = sys.modules.get("torch")
+    if torch is None:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:  # a CPU-only PyTorch: nothing to order
+        pass
+
+
 def lib():
     """Load (once) and return the shared library with typed signatures."""
     global _lib
@@ -52,6 +70,7 @@ def lib():
         return _lib
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m kwave_amd._build` (hipcc, gfx950)")
+    _torch_context_first()
     L = C.CDLL(str(LIB_PATH))
     vp, i32, i64, f32p, f64p, i64p = C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int64)
     sig = {
@@ -74,6 +93,8 @@ def lib():
         "kw_spectral_apply": ([i32, C.POINTER(C.c_int), f64p, i32, C.c_double, i32, i32, C.c_double, f32p, f32p, i32], i32),
         "kw_plan_run_profiled": ([vp, i64, f64p, f64p, i64p], i32),
         "kw_plan_adjoint_c": ([vp, f32p, i32, f32p], i32),
+        "kw_mode_gemm_bf16": ([i32, i32, i32, i32, vp, vp, vp, vp], i32),
+        "kw_mode_contract_bf16": ([i32, i32, i32, i32, i32, vp, i64p, vp, vp, i64p, vp], i32),
         "kw_kernel_kind_count": ([], i32),
         "kw_kernel_kind_name": ([i32], C.c_char_p),
     }

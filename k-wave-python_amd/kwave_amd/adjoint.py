@@ -57,6 +57,9 @@ def sound_speed_gradient(kgrid, medium, source, sensor, adj_src, *, c=None, pml_
 def _autograd():
     import torch
 
+    if torch.cuda.is_available():
+        torch.cuda.init()  # PyTorch's HIP context before the solver library's (see _abi._torch_context_first)
+
     class SensorTraces(torch.autograd.Function):
         """c0 map -> recorded p traces; backward runs the reverse-time adjoint."""
 

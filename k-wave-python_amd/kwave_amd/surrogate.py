@@ -1,0 +1,449 @@
+"""Fourier-neural-operator surrogate: sound-speed map -> sensor traces, trained on solver data.
+
+The problem is ``workloads.surrogate_2d``: a 128^2 c map in [1400, 1650] m/s, a
+32-cell additive line source with a coincident 32-cell receive line, 600 steps.
+Training data come from the GPU solver (batched plans); the solver is also the
+judge of the surrogate's error and its fallback.
+
+Network (runs at 64^2, the c map averaged over 2 x 2 cells):
+  lift     (c, x, y) -> 16 channels, pointwise
+  3 x      v <- GELU(SpectralConv(v) + W v + b)   (16 x 16 retained modes, both kx signs)
+  head     for every sensor, the feature columns at its 64^2 column and the
+           neighbour on its side (2 x 16 channels x 64 depths) and the full-resolution
+           c column under the sensor (128 depths; the injected amplitude scales with
+           the local c) -> 600 samples by one shared linear map, plus a learned
+           per-sensor mean trace.
+
+The spectral contraction of every layer - forward, input gradient and weight
+gradient - runs in the library's BF16 MFMA kernel (kw_mode_gemm_bf16, FP32
+accumulation); FFTs, pointwise layers and the optimiser are PyTorch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _abi, workloads
+
+N_FINE = 128
+N_COARSE = 64
+MODES = 16
+WIDTH = 16
+LAYERS = 3
+TRACE_SCALE = 1e5  # Pa; traces are learned in units of the source amplitude
+C_MID, C_HALF = 1525.0, 125.0
+
+
+# ---------------------------------------------------------------------------
+# the BF16 MFMA contraction
+# ---------------------------------------------------------------------------
+def mode_gemm(a, bt):
+    """C[m] = A[m] @ Bt[m]^T on the matrix cores; a: (M, R, K), bt: (M, N, K) bfloat16 CUDA tensors."""
+    import torch
+
+    if a.dtype != torch.bfloat16 or bt.dtype != torch.bfloat16 or not a.is_cuda:
+        raise ValueError("mode_gemm takes bfloat16 device tensors")
+    a, bt = a.contiguous(), bt.contiguous()
+    m, r, k = a.shape
+    n = bt.shape[1]
+    if bt.shape[0] != m or bt.shape[2] != k:
+        raise ValueError(f"mode_gemm shapes {tuple(a.shape)} x {tuple(bt.shape)}")
+    out = torch.empty((m, r, n), dtype=torch.float32, device=a.device)
+    stream = torch.cuda.current_stream(a.device).cuda_stream
+    _abi.check(_abi.lib().kw_mode_gemm_bf16(m, r, n, k, C.c_void_p(a.data_ptr()), C.c_void_p(bt.data_ptr()),
+                                            C.c_void_p(out.data_ptr()), C.c_void_p(stream)))
+    return out
+
+
+def _contract_fn():
+    import torch
+
+    class SpectralContract(torch.autograd.Function):
+        """Y2[m] = X2[m] W2[m] with real-stacked complex modes (see oracle/fno_ref.py for the algebra)."""
+
+        @staticmethod
+        def forward(ctx, x2, w2):
+            xb = x2.to(torch.bfloat16).contiguous()
+            wb = w2.to(torch.bfloat16).contiguous()
+            ctx.save_for_backward(xb, wb)
+            return mode_gemm(xb, wb.transpose(1, 2))
+
+        @staticmethod
+        def backward(ctx, g2):
+            xb, wb = ctx.saved_tensors
+            gb = g2.to(torch.bfloat16).contiguous()
+            dx = mode_gemm(gb, wb)                                              # G2 W2^T
+            dw = mode_gemm(xb.transpose(1, 2), gb.transpose(1, 2))              # X2^T G2
+            return dx, dw
+
+    return SpectralContract
+
+
+def mode_contract(x, bt, md):
+    """Spectrum (complex64, [B, C, NX, NY]) -> zeros except the retained modes, contracted with bt.
+
+    bt: bf16 [2 md md, 2C, 2C], row n = output index (interleaved re/im), i.e. the
+    transposed real weight matrix of the forward pass or the plain one of the
+    input gradient.  Runs kw_mode_contract_bf16 (gather + MFMA + scatter fused).
+    """
+    import torch
+
+    b, c, nx, ny = x.shape
+    if x.dtype != torch.complex64 or bt.dtype != torch.bfloat16 or not bt.is_contiguous():
+        raise ValueError("mode_contract takes a complex64 spectrum and contiguous bf16 weights")
+    if tuple(bt.shape) != (2 * md * md, 2 * c, 2 * c):
+        raise ValueError(f"weights {tuple(bt.shape)} do not match {2 * md * md} modes of {c} channels")
+    y = torch.view_as_complex(torch.zeros((b, c, nx, ny, 2), dtype=torch.float32, device=x.device))
+    xs = (C.c_int64 * 4)(*x.stride())
+    ys = (C.c_int64 * 4)(*y.stride())
+    stream = torch.cuda.current_stream(x.device).cuda_stream
+    _abi.check(_abi.lib().kw_mode_contract_bf16(b, c, md, nx, ny, C.c_void_p(x.data_ptr()), xs, C.c_void_p(bt.data_ptr()),
+                                                C.c_void_p(y.data_ptr()), ys, C.c_void_p(stream)))
+    return y
+
+
+def gather_modes(x, md):
+    """Retained modes of a spectrum as interleaved real pairs, [2 md md, B, 2C]."""
+    import torch
+
+    b, c, nx, _ = x.shape
+    kx = torch.cat([torch.arange(md), torch.arange(nx - md, nx)]).to(x.device)
+    sel = x.index_select(2, kx)[..., :md]
+    return torch.view_as_real(sel.permute(2, 3, 0, 1)).reshape(2 * md * md, b, 2 * c)
+
+
+def _modes_fn():
+    import torch
+    import torch.nn.functional as F
+
+    class SpectralModes(torch.autograd.Function):
+        """out = spectrum whose retained modes are X W (per mode), zeros elsewhere."""
+
+        @staticmethod
+        def forward(ctx, ft, w2, md):
+            wb = w2.to(torch.bfloat16).contiguous()
+            ctx.save_for_backward(ft, wb)
+            ctx.md = md
+            return mode_contract(ft, wb.transpose(1, 2).contiguous(), md)
+
+        @staticmethod
+        def backward(ctx, g):
+            ft, wb = ctx.saved_tensors
+            md = ctx.md
+            dft = mode_contract(g, wb, md)                                       # G2 W2^T at the modes
+            b = ft.shape[0]
+            bp = (b + 7) // 8 * 8
+            xt = gather_modes(ft, md).to(torch.bfloat16).transpose(1, 2)
+            gt = gather_modes(g, md).to(torch.bfloat16).transpose(1, 2)
+            if bp != b:
+                xt, gt = F.pad(xt, (0, bp - b)), F.pad(gt, (0, bp - b))
+            dw = mode_gemm(xt.contiguous(), gt.contiguous())                     # X2^T G2
+            return dft, dw, None
+
+    return SpectralModes
+
+
+_MODES = None
+
+
+def spectral_modes(ft, w2, md):
+    global _MODES
+    if _MODES is None:
+        _MODES = _modes_fn()
+    return _MODES.apply(ft, w2, md)
+
+
+_CONTRACT = None
+
+
+def spectral_contract(x2, w2):
+    global _CONTRACT
+    if _CONTRACT is None:
+        _CONTRACT = _contract_fn()
+    return _CONTRACT.apply(x2, w2)
+
+
+# ---------------------------------------------------------------------------
+# the network
+# ---------------------------------------------------------------------------
+def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
+    import torch
+    import torch.nn as nn
+    import torch.nn.functional as F
+
+    torch.manual_seed(seed)
+
+    class SpectralConv2d(nn.Module):
+        """Retained modes kx in [0, m) and [n - m, n), ky in [0, m), contracted per mode.
+
+        Real and imaginary parts are interleaved per channel (x2[..., 2i + r]), so the
+        FFT output reaches the contraction through one gather and a view; the weight
+        matrix has the matching interleaved block pattern
+        W2[2i + a, 2o + b] = [[Re W, Im W], [-Im W, Re W]][a][b].
+        """
+
+        def __init__(self, c, modes, n):
+            super().__init__()
+            self.c, self.modes = c, modes
+            scale = 1.0 / (c * c)
+            self.wr = nn.Parameter(scale * torch.randn(2 * modes * modes, c, c))
+            self.wi = nn.Parameter(scale * torch.randn(2 * modes * modes, c, c))
+            self.register_buffer("kx", torch.cat([torch.arange(modes), torch.arange(n - modes, n)]))
+
+        def weights(self):
+            c = self.c
+            w2 = torch.stack([torch.stack([self.wr, self.wi], -1), torch.stack([-self.wi, self.wr], -1)], 2)
+            return w2.reshape(-1, 2 * c, 2 * c)
+
+        def forward(self, v):
+            n = v.shape[-1]
+            ft = torch.fft.rfft2(v.float())                                      # (b, c, n, n/2+1)
+            return torch.fft.irfft2(spectral_modes(ft, self.weights(), self.modes), s=(n, n))
+
+    class FNO(nn.Module):
+        def __init__(self, n_sens, nt, cols_a, cols_b, cols_f):
+            super().__init__()
+            self.width, self.layers = width, layers
+            # default-scale initialisation (uniform, bound 1/sqrt(fan_in)); the extra head terms start at zero
+            self.lift_c = nn.Parameter(torch.empty(width).uniform_(-3 ** -0.5, 3 ** -0.5))
+            self.lift_g = nn.Conv2d(2, width, 1)
+            self.spec = nn.ModuleList([SpectralConv2d(width, modes, N_COARSE) for _ in range(layers)])
+            self.pw = nn.ParameterList([nn.Parameter(torch.empty(width, width).uniform_(-width ** -0.5, width ** -0.5))
+                                        for _ in range(layers)])
+            self.pb = nn.ParameterList([nn.Parameter(torch.zeros(width, 1)) for _ in range(layers)])
+            self.head_a = nn.Linear(width * N_COARSE, nt)
+            self.head_b = nn.Linear(width * N_COARSE, nt, bias=False)
+            self.head_f = nn.Linear(N_FINE, nt, bias=False)
+            nn.init.zeros_(self.head_b.weight)
+            nn.init.zeros_(self.head_f.weight)
+            self.mean = nn.Parameter(torch.zeros(n_sens, nt))
+            self.register_buffer("cols_ab", torch.as_tensor(np.concatenate([cols_a, cols_b]), dtype=torch.long))
+            self.register_buffer("cols_f", torch.as_tensor(cols_f, dtype=torch.long))
+            g = torch.linspace(0, 1, N_COARSE)
+            self.register_buffer("grid", torch.stack(torch.meshgrid(g, g, indexing="ij"))[None])
+            self.n_sens = n_sens
+
+        def forward(self, cmap):
+            b = cmap.shape[0]
+            n = N_COARSE
+            cn = (cmap - C_MID) / C_HALF
+            x = F.avg_pool2d(cn[:, None], 2)                                     # (b, 1, n, n)
+            v = x * self.lift_c[None, :, None, None] + self.lift_g(self.grid)    # lift of (c, x, y)
+            wd = self.width
+            for i in range(self.layers):
+                w = torch.matmul(self.pw[i], v.reshape(b, wd, n * n)) + self.pb[i]
+                v = self.spec[i](v) + w.reshape(b, wd, n, n)
+                if i < self.layers - 1:
+                    v = F.gelu(v)
+            sel = v.index_select(3, self.cols_ab).permute(0, 3, 1, 2).reshape(b, 2, self.n_sens, wd * n)
+            ff = cn.index_select(2, self.cols_f).permute(0, 2, 1)                # (b, n_sens, 128)
+            return self.mean[None] + self.head_a(sel[:, 0]) + self.head_b(sel[:, 1]) + self.head_f(ff)
+
+    kg, med, src, sen, pml, alpha = workloads.surrogate_2d(0)
+    ys = np.where(np.asarray(sen.mask).any(axis=0))[0]
+    cols_a = ys // 2
+    cols_b = np.clip(cols_a + np.where(ys % 2 == 1, 1, -1), 0, N_COARSE - 1)
+    return FNO(len(ys), kg.Nt, cols_a, cols_b, ys).to(device)
+
+
+# ---------------------------------------------------------------------------
+# data from the solver
+# ---------------------------------------------------------------------------
+def solve_batch(cmaps, plan_batch=256):
+    """Recorded traces (n, 32, 600) float32 of the surrogate problem for each c map, batched on the GPU."""
+    _torch_first()
+    from .solvers.plan import DevicePlan
+    from .solvers.setup import build_instance
+
+    out = []
+    for s in range(0, len(cmaps), plan_batch):
+        sts = []
+        for c in cmaps[s:s + plan_batch]:
+            kg, med, src, sen, pml, alpha = workloads.surrogate_2d(0, c=c)
+            sts.append(build_instance(kg, med, src, sen, pml_size=pml, pml_alpha=alpha))
+        plan = DevicePlan(sts)
+        try:
+            plan.run(sts[0].Nt - plan.step_index(0))
+            out.extend(plan.recorded(b, _abi.F_REC_P).copy() for b in range(len(sts)))
+        finally:
+            plan.close()
+    return np.stack(out).astype(np.float32)
+
+
+def make_dataset(seeds, plan_batch=256):
+    cmaps = np.stack([workloads.surrogate_c_map(int(s)) for s in seeds]).astype(np.float32)
+    return cmaps, solve_batch(cmaps.astype(np.float64), plan_batch)
+
+
+def rel_l2(pred, true):
+    """Per-sample relative L2 error of trace sets (n, 32, 600)."""
+    d = (pred - true).reshape(len(true), -1)
+    return np.linalg.norm(d, axis=1) / np.linalg.norm(true.reshape(len(true), -1), axis=1)
+
+
+# ---------------------------------------------------------------------------
+# training and inference
+# ---------------------------------------------------------------------------
+def train(model, cmaps, traces, epochs=40, batch=64, lr=2e-3, log=None):
+    """Minimise the mean per-sample relative L2 of the traces with Adam (cosine schedule)."""
+    import torch
+
+    dev = next(model.parameters()).device
+    x = torch.as_tensor(cmaps, device=dev)
+    y = torch.as_tensor(traces / TRACE_SCALE, device=dev)
+    with torch.no_grad():
+        model.mean.copy_(y.mean(0))
+    opt = torch.optim.Adam(model.parameters(), lr=lr)
+    steps = epochs * ((len(x) + batch - 1) // batch)
+    sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=lr, total_steps=steps, pct_start=0.1)
+    g = torch.Generator(device="cpu").manual_seed(0)
+    hist = []
+    for ep in range(epochs):
+        perm = torch.randperm(len(x), generator=g).to(dev)
+        tot = 0.0
+        for i in range(0, len(x), batch):
+            idx = perm[i:i + batch]
+            pred = model(x[idx])
+            t = y[idx]
+            loss = ((pred - t).flatten(1).norm(dim=1) / t.flatten(1).norm(dim=1)).mean()
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            opt.step()
+            sched.step()
+            tot += float(loss.detach()) * len(idx)
+        hist.append(tot / len(x))
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: train rel-L2 {hist[-1]:.4f}")
+    return hist
+
+
+def predict(model, cmaps, batch=512):
+    """Surrogate traces (n, 32, 600) in Pa (pointwise layers and head in bf16 autocast)."""
+    import torch
+
+    dev = next(model.parameters()).device
+    out = []
+    with torch.no_grad(), torch.autocast("cuda", dtype=torch.bfloat16):
+        for i in range(0, len(cmaps), batch):
+            out.append(model(torch.as_tensor(cmaps[i:i + batch], device=dev)).float())
+    return (torch.cat(out) * TRACE_SCALE).cpu().numpy()
+
+
+class Surrogate:
+    """Trained network with the solver as fallback.
+
+    ``__call__`` returns traces for c maps; maps outside the training range
+    [1400, 1650] m/s go to the solver.  ``validate`` measures the error on
+    solver-checked maps; if the measured mean error exceeds ``max_err`` (2 %),
+    the surrogate disables itself and every later call goes to the solver.
+    """
+
+    def __init__(self, model, max_err=0.02):
+        self.model, self.max_err, self.enabled = model, max_err, True
+
+    def validate(self, cmaps, traces=None):
+        traces = solve_batch(np.asarray(cmaps, dtype=np.float64)) if traces is None else traces
+        err = rel_l2(predict(self.model, np.asarray(cmaps, dtype=np.float32)), traces)
+        self.enabled = bool(err.mean() <= self.max_err)
+        return err
+
+    def __call__(self, cmaps):
+        cmaps = np.asarray(cmaps, dtype=np.float32)
+        lo, hi = workloads.SURROGATE_C_RANGE
+        ood = (cmaps.reshape(len(cmaps), -1).min(1) < lo - 1e-3) | (cmaps.reshape(len(cmaps), -1).max(1) > hi + 1e-3)
+        if not self.enabled:
+            ood[:] = True
+        out = np.empty((len(cmaps),) + tuple(self.model.mean.shape), dtype=np.float32)
+        if (~ood).any():
+            out[~ood] = predict(self.model, cmaps[~ood])
+        if ood.any():
+            out[ood] = solve_batch(cmaps[ood].astype(np.float64))
+        return out, ood
+
+
+def time_surrogate(model, cmaps, batch=1024, reps=5):
+    """Device time per sample of the surrogate forward (inputs already on the device)."""
+    import torch
+
+    x = torch.as_tensor(cmaps, device=next(model.parameters()).device)
+    xb = x.repeat((batch + len(x) - 1) // len(x), 1, 1)[:batch].contiguous()
+    with torch.no_grad(), torch.autocast("cuda", dtype=torch.bfloat16):
+        model(xb)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            model(xb)
+        torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / (reps * batch)
+
+
+def time_solver(cmaps):
+    """Device time per sample of the batched solver's time loop (plan uploaded, graph captured)."""
+    from .solvers.plan import DevicePlan
+    from .solvers.setup import build_instance
+
+    sts = []
+    for c in cmaps:
+        kg, med, src, sen, pml, alpha = workloads.surrogate_2d(0, c=c)
+        sts.append(build_instance(kg, med, src, sen, pml_size=pml, pml_alpha=alpha))
+    plan = DevicePlan(sts)
+    try:
+        plan.run(sts[0].Nt)
+        plan.reset()
+        plan.run(0)
+        t0 = time.perf_counter()
+        plan.run(sts[0].Nt)
+        return (time.perf_counter() - t0) / len(sts)
+    finally:
+        plan.close()
+
+
+def _torch_first():
+    """Initialise PyTorch's GPU context before the solver library touches the device.
+
+    torch and the solver library each load a HIP runtime; initialising torch
+    first is the order that has always worked (once, torch failed to find the
+    GPU when the solver had run first in the same process).
+    """
+    import torch
+
+    torch.cuda.init()
+
+
+def benchmark(n_train=4096, n_test=512, epochs=80, log=print):
+    """Train on solver data, then report held-out error and surrogate vs batched-solver throughput."""
+    _torch_first()
+    t0 = time.perf_counter()
+    xtr, ytr = make_dataset(5_000_000 + np.arange(n_train))
+    t_gen = time.perf_counter() - t0
+    xte, yte = make_dataset(6_000_000 + np.arange(n_test))
+    log(f"surrogate: {n_train} training samples generated in {t_gen:.1f} s")
+    model = build_model()
+    t0 = time.perf_counter()
+    train(model, xtr, ytr, epochs=epochs, log=log)
+    t_train = time.perf_counter() - t0
+    pred = predict(model, xte)
+    err = rel_l2(pred, yte)
+    base = rel_l2(np.broadcast_to(ytr.mean(0), yte.shape), yte)
+    # throughput, inputs resident on the device: surrogate forward vs the batched solver's time loop
+    t_sur = time_surrogate(model, xte)
+    t_sol = time_solver(xte.astype(np.float64))
+    # host to host: c maps in host memory -> traces in host memory (solver incl. its host set-up)
+    t0 = time.perf_counter()
+    predict(model, xte)
+    t_sur_h = (time.perf_counter() - t0) / n_test
+    t0 = time.perf_counter()
+    solve_batch(xte.astype(np.float64), plan_batch=n_test)
+    t_sol_h = (time.perf_counter() - t0) / n_test
+    res = {"n_train": n_train, "n_test": n_test, "epochs": epochs, "train_s": t_train, "datagen_s": t_gen,
+           "test_rel_l2_mean": float(err.mean()), "test_rel_l2_p95": float(np.percentile(err, 95)),
+           "test_frac_above_2pct": float((err > 0.02).mean()), "mean_trace_baseline_rel_l2": float(base.mean()),
+           "surrogate_ms_per_sample": 1e3 * t_sur, "solver_ms_per_sample_batched": 1e3 * t_sol,
+           "speedup_vs_batched_solver": t_sol / t_sur,
+           "host_to_host_surrogate_ms_per_sample": 1e3 * t_sur_h, "host_to_host_solver_ms_per_sample": 1e3 * t_sol_h,
+           "host_to_host_speedup": t_sol_h / t_sur_h}
+    log(f"surrogate: held-out rel-L2 {err.mean():.4f} (mean-trace baseline {base.mean():.4f}), "
+        f"{1e3 * t_sur:.4f} ms/sample vs solver {1e3 * t_sol:.3f} ms/sample -> {t_sol / t_sur:.0f}x")
+    return res, model

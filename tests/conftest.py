@@ -19,3 +19,18 @@ def kwlib():
     from kwave_amd import _abi
 
     return _abi.lib()
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _torch_gpu_context_first(request):
+    """GPU sessions initialise PyTorch's HIP context before the solver library loads.
+
+    PyTorch carries its own HIP runtime; it finds the GPU only if it initialises
+    before the system runtime the solver library uses.
+    """
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    yield

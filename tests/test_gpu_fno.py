@@ -1,0 +1,123 @@
+"""The surrogate's BF16 MFMA spectral contraction and training, on the GPU.
+
+* kw_mode_gemm_bf16 is exact on integer-valued bf16 operands (this also fixes the
+  MFMA lane layout the kernel assumes);
+* forward, input gradient and weight gradients of the spectral contraction match
+  the FP64 CPU reference (oracle/fno_ref.py) on the same bf16-rounded operands;
+* a short training run on solver data beats the method without the network
+  (predicting the training-mean traces) on held-out maps;
+* the surrogate hands out-of-range maps to the solver and disables itself when
+  validation shows more than the allowed error.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def test_mode_gemm_exact_on_integers():
+    import torch
+
+    from kwave_amd.surrogate import mode_gemm
+
+    g = torch.Generator().manual_seed(0)
+    m, r, n, k = 3, 64, 96, 40
+    a = torch.randint(-8, 9, (m, r, k), generator=g).float()
+    bt = torch.randint(-8, 9, (m, n, k), generator=g).float()
+    c = mode_gemm(a.to("cuda", torch.bfloat16), bt.to("cuda", torch.bfloat16)).cpu()
+    want = torch.bmm(a.double(), bt.double().transpose(1, 2))
+    assert torch.equal(c.double(), want)
+
+
+def test_spectral_contract_matches_fp64_reference():
+    import torch
+
+    from kwave_amd.surrogate import spectral_contract
+    from oracle.fno_ref import bf16, contract, contract_grads, stack_weights, weight_grads
+
+    rng = np.random.default_rng(1)
+    m, b, c = 12, 64, 32
+    x2 = rng.standard_normal((m, b, 2 * c)).astype(np.float32)
+    wr = (0.1 * rng.standard_normal((m, c, c))).astype(np.float32)
+    wi = (0.1 * rng.standard_normal((m, c, c))).astype(np.float32)
+    g2 = rng.standard_normal((m, b, 2 * c)).astype(np.float32)
+    xt = torch.tensor(x2, device="cuda", requires_grad=True)
+    wrt = torch.tensor(wr, device="cuda", requires_grad=True)
+    wit = torch.tensor(wi, device="cuda", requires_grad=True)
+    w2t = torch.cat([torch.cat([wrt, wit], 2), torch.cat([-wit, wrt], 2)], 1)
+    y2 = spectral_contract(xt, w2t)
+    y2.backward(torch.tensor(g2, device="cuda"))
+    xb, wb, gb = bf16(x2), bf16(stack_weights(wr, wi)), bf16(g2)
+    y_ref = contract(xb, wb)
+    dx_ref, dw2_ref = contract_grads(xb, wb, gb)
+    dwr_ref, dwi_ref = weight_grads(dw2_ref, c)
+    errs = [_rel(y2.detach().cpu().numpy(), y_ref), _rel(xt.grad.cpu().numpy(), dx_ref),
+            _rel(wrt.grad.cpu().numpy(), dwr_ref), _rel(wit.grad.cpu().numpy(), dwi_ref)]
+    print("forward, dX, dRe W, dIm W rel-L2 vs FP64:", ["%.1e" % e for e in errs])
+    assert max(errs) < 1e-5, errs
+
+
+def test_surrogate_training_beats_baseline_and_falls_back():
+    from kwave_amd import surrogate, workloads
+
+    xtr, ytr = surrogate.make_dataset(5_000_000 + np.arange(1024))
+    xte, yte = surrogate.make_dataset(6_000_000 + np.arange(128))
+    model = surrogate.build_model()
+    hist = surrogate.train(model, xtr, ytr, epochs=30, log=print)
+    assert hist[-1] < hist[0]
+    err = surrogate.rel_l2(surrogate.predict(model, xte), yte)
+    base = surrogate.rel_l2(np.broadcast_to(ytr.mean(0), yte.shape), yte)
+    print(f"held-out rel-L2: surrogate {err.mean():.4f}, mean-trace baseline {base.mean():.4f}")
+    assert err.mean() < 0.7 * base.mean(), (err.mean(), base.mean())
+
+    sur = surrogate.Surrogate(model)
+    odd = xte[:2].copy()
+    odd[1, 40:50, 60:70] = workloads.SURROGATE_C_RANGE[1] + 50.0  # outside the training range
+    out, used_solver = sur(odd)
+    assert used_solver.tolist() == [False, True]
+    assert np.array_equal(out[1], surrogate.solve_batch(odd[1:].astype(np.float64))[0])
+    strict = surrogate.Surrogate(model, max_err=1e-9)
+    strict.validate(xte[:4], yte[:4])
+    assert not strict.enabled
+    out, used_solver = strict(xte[:2])
+    assert used_solver.all()
+
+
+def test_fused_mode_contraction_matches_gather_path():
+    """kw_mode_contract_bf16 (gather + MFMA + scatter fused) == explicit gather + kw_mode_gemm_bf16 + scatter,
+    forward and gradients, bit for bit (same bf16 operands, same accumulation order)."""
+    import torch
+
+    from kwave_amd.surrogate import gather_modes, spectral_contract, spectral_modes
+
+    torch.manual_seed(3)
+    b, c, n, md = 40, 16, 64, 16
+    v = torch.randn(b, c, n, n, device="cuda")
+    wr = (0.05 * torch.randn(2 * md * md, c, c, device="cuda")).requires_grad_()
+    wi = (0.05 * torch.randn(2 * md * md, c, c, device="cuda")).requires_grad_()
+
+    def w2():
+        return torch.stack([torch.stack([wr, wi], -1), torch.stack([-wi, wr], -1)], 2).reshape(-1, 2 * c, 2 * c)
+
+    ft = torch.fft.rfft2(v).requires_grad_()
+    g = torch.randn(b, c, n, n // 2 + 1, dtype=torch.complex64, device="cuda")
+    out = spectral_modes(ft, w2(), md)
+    (torch.view_as_real(out) * torch.view_as_real(g)).sum().backward()
+    fused = [out.detach().clone(), ft.grad.clone(), wr.grad.clone(), wi.grad.clone()]
+    ft.grad = wr.grad = wi.grad = None
+
+    x2 = torch.nn.functional.pad(gather_modes(ft, md), (0, 0, 0, 64 - b))
+    y2 = spectral_contract(x2, w2())[:, :b]
+    y = torch.view_as_complex(y2.reshape(2 * md, md, b, c, 2)).permute(2, 3, 0, 1)
+    kx = torch.cat([torch.arange(md), torch.arange(n - md, n)]).cuda()
+    out_ref = torch.zeros(b, c, n, n // 2 + 1, dtype=torch.complex64, device="cuda")
+    out_ref[:, :, kx, :md] = y
+    (torch.view_as_real(out_ref) * torch.view_as_real(g)).sum().backward()
+    assert torch.equal(fused[0], out_ref.detach())
+    assert torch.equal(fused[1], ft.grad)
+    for a, bb in zip(fused[2:], (wr.grad, wi.grad)):
+        assert torch.allclose(a, bb, rtol=0, atol=1e-6 * float(bb.abs().max())), float((a - bb).abs().max())
