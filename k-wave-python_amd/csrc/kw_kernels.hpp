@@ -736,6 +736,32 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COL
   int iy, iz;
   if (g.nd == 2) { iy = valid ? col : 0; iz = 0; }
   else { iy = o; iz = valid ? col : 0; }
+  // Operator values read between the transforms come from LDS, so no global
+  // latency sits between a forward and an inverse transform: the x wavenumbers,
+  // the x-axis operator vectors of up to two tabled jobs, and per column of the
+  // tile the value of every y/z-axis job.  (The twiddle barrier above has passed;
+  // the first transform's barriers publish these before they are read.)
+  __shared__ float kxs[N];
+  __shared__ float2 xop[2][N];
+  __shared__ float2 cop[kMaxJobs][L];
+  for (int i_ = tid; i_ < N; i_ += C::NT) kxs[i_] = gp(ops.k[0])[i_];
+  for (int j = 0, slot = 0; j < a.njob; ++j) {
+    const XJob jb = a.job[j];
+    const float2* t = jb.kind == XK_GRAD ? ops.grad[jb.axis] : jb.kind == XK_DIV ? ops.div[jb.axis]
+                    : jb.kind == XK_UNSTAGGER ? ops.unst[jb.axis] : nullptr;
+    if (!t) continue;
+    if (jb.axis == 0) {
+      if (slot < 2)
+        for (int i_ = tid; i_ < N; i_ += C::NT) xop[slot][i_] = ld2(gp(t), i_);
+      ++slot;
+    } else {
+      for (int cc = tid; cc < L; cc += C::NT) {
+        const int colc = min(tile * L + cc, a.ncols - 1);
+        const int ia = (jb.axis == 1 && g.nd == 3) ? o : colc;
+        cop[j][cc] = ld2(gp(t), ia);
+      }
+    }
+  }
   float kyz2 = 0.f;
   {
     const float ky = gp(ops.k[1])[iy];
@@ -759,22 +785,30 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COL
       for (int m = 0; m < C::E; ++m) st[m] = valid ? ld2(src, base + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
       fft_reg<N, -1, typename C::Idx, EM>(st, lds, tw, c, j0, idx);
     }
-    const int ixa = jb.axis == 0 ? -1 : (jb.axis == 1 ? iy : iz);
+    // where this job's operator values live: an x-table slot, the per-column
+    // values, or (a third x-axis table) global memory
+    const bool tabled = jb.kind == XK_GRAD || jb.kind == XK_DIV || jb.kind == XK_UNSTAGGER;
+    int slot = 0;
+    for (int i = 0; i < j; ++i) {
+      const XJob q = a.job[i];
+      slot += q.axis == 0 && (q.kind == XK_GRAD || q.kind == XK_DIV || q.kind == XK_UNSTAGGER);
+    }
+    const float2* gtab = jb.kind == XK_GRAD ? ops.grad[0] : jb.kind == XK_DIV ? ops.div[0] : ops.unst[0];
+    const float2 colv = (tabled && jb.axis != 0) ? cop[j][c] : make_float2(1.f, 0.f);
 #pragma unroll
     for (int m = 0; m < C::E; ++m) {
       const int n = j0 + m * C::TPL;
-      const int ia = ixa < 0 ? n : ixa;
-      const float kx = gp(ops.k[0])[n];
+      const float kx = kxs[n];
       const float km = sqrtf(fmaf(kx, kx, kyz2));
-      float2 op;
+      float2 op = make_float2(1.f, 0.f);
+      if (tabled) op = jb.axis != 0 ? colv : (slot < 2 ? xop[slot][n] : ld2(gp(gtab), n));
       switch (jb.kind) {
-        case XK_GRAD: op = cscale(ld2(gp(ops.grad[jb.axis]), ia), kap_m[m]); break;
-        case XK_DIV: op = cscale(ld2(gp(ops.div[jb.axis]), ia), kap_m[m]); break;
+        case XK_GRAD:
+        case XK_DIV: op = cscale(op, kap_m[m]); break;
         case XK_FRAC1: op = make_float2(kpow(km, ops.frac1), 0.f); break;
         case XK_FRAC2: op = make_float2(kpow(km, ops.frac2), 0.f); break;
         case XK_SRCKAPPA: op = make_float2(kap > 0.f ? cos_k(kap * km) : 1.f, 0.f); break;
-        case XK_UNSTAGGER: op = ld2(gp(ops.unst[jb.axis]), ia); break;
-        default: op = make_float2(1.f, 0.f); break;
+        default: break;
       }
       v[m] = cscale(cmul(st[m], op), ops.inv_ntot);
     }
