@@ -86,11 +86,25 @@ __device__ __forceinline__ void load_pair_spec(float2* lds, const float2* __rest
   using C = ZCfg<N>;
   const auto spec = gp(spec_);
   const typename C::Idx idx{};
-  for (int e = tid; e < C::L * C::NH; e += C::NT) {
+  // all of this thread's loads are issued before the first LDS write, so their
+  // latencies overlap instead of adding up
+  constexpr int IT = (C::L * C::NH + C::NT - 1) / C::NT;
+  float2 As[IT], Bs[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int e = tid + it * C::NT;
     const int p = e / C::NH, k = e % C::NH;
     const int ra = row0 + 2 * p, rb = ra + 1;
-    float2 A = ra < rows ? ld2(spec, (int64_t)ra * pitch + k) : make_float2(0.f, 0.f);
-    float2 B = rb < rows ? ld2(spec, (int64_t)rb * pitch + k) : make_float2(0.f, 0.f);
+    const bool in = e < C::L * C::NH;
+    As[it] = in && ra < rows ? ld2(spec, (int64_t)ra * pitch + k) : make_float2(0.f, 0.f);
+    Bs[it] = in && rb < rows ? ld2(spec, (int64_t)rb * pitch + k) : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int e = tid + it * C::NT;
+    if (e >= C::L * C::NH) break;
+    const int p = e / C::NH, k = e % C::NH;
+    const float2 A = As[it], B = Bs[it];
     if (k == 0 || (N % 2 == 0 && k == N / 2)) {
       lds[idx(k, p)] = make_float2(A.x, B.x);
     } else {
@@ -198,15 +212,16 @@ __device__ __forceinline__ void set_lane(float2 (&a)[E], int m, int h, float v) 
 }
 
 // Minimum waves per SIMD the register allocator must leave room for (kernels
-// with short register arrays only; 1 = unconstrained).  Measured at 512^3: the
-// mass kernel runs 2668 us with 3 waves (168 VGPRs, a few spills) against 2901 us
-// with 2; the momentum kernel is fastest unconstrained (1904 us at 3 waves, 2808
-// us when forced to 4).
+// with short register arrays only; 1 = unconstrained).  Measured at 512^3: every
+// floor that forces spills loses.  With the batched spectrum loads the mass kernel
+// runs 2409 us unconstrained (2 waves) against 2595 us forced to 3 (29 spilled
+// registers); the momentum kernel ran 1904 us at its natural 3 waves and 2808 us
+// forced to 4.
 #ifndef KW_WPE_MOM
 #define KW_WPE_MOM 1
 #endif
 #ifndef KW_WPE_MASS
-#define KW_WPE_MASS 3
+#define KW_WPE_MASS 1
 #endif
 #ifndef KW_WPE_EOS
 #define KW_WPE_EOS 1
