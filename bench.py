@@ -252,6 +252,7 @@ def main():
                    "bytes_per_launch": v[1] / v[2]} for k, v in kern.items()}
     alg_bytes_step = sum(v[1] for v in kern.values()) / steps
     gpu_ms_step = sum(v[0] for v in kern.values()) / steps
+    tiles = plan.tiles()
     p_rec = plan.read(0, _abi.F_REC_P)
     finite = bool(abs(p_rec).max() < 1e12)
     out = {
@@ -277,6 +278,7 @@ def main():
         "alg_bytes_per_voxel_step": alg_bytes_step / ntot,
         "gpu_ms_per_step": gpu_ms_step,
         "kernels": kernels,
+        "column_tiles": dict(zip(("fft_only", "spectral_jobs"), tiles)),
         "finite": finite,
     }
     plan.close()

@@ -140,6 +140,11 @@ int kw_plan_sync(kw_plan* plan);
 int kw_plan_run_profiled(kw_plan* plan, int64_t nsteps, double* kernel_ms, double* kernel_bytes, int64_t* launches);
 int kw_kernel_kind_count(void);
 const char* kw_kernel_kind_name(int kind);
+/* Tile-width variants the plan picked by timing at its first kw_plan_init (0:
+ * the default width, 1: half of it) for the FFT-only column passes and for the
+ * passes with spectral jobs; KW_AUTOTUNE=0 in the environment keeps 0.  No
+ * counterpart in the reference (its FFTs are library calls). */
+int kw_plan_tiles(kw_plan* plan, int* fft_tile, int* job_tile);
 /* Copy a device field of instance b to host memory (n floats). */
 int kw_plan_read(kw_plan* plan, int b, int which, float* out, int64_t n);
 /* Number of floats kw_plan_read returns for `which`. */

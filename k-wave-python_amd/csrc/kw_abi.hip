@@ -151,6 +151,8 @@ struct kw_plan {
   int graph_k = 8;
   bool graphs = true;
   bool skip_gather = false;  // replayed steps of the reverse sweep do not record
+  int col_tile[2] = {0, 0};  // tile variant of the FFT-only and the spectral-job column passes
+  bool tuned = false;
   bool inst_dirty = true;
   // per-kernel timing (kw_plan_run_profiled)
   bool prof = false;
@@ -229,6 +231,7 @@ int ev_get(kw_plan* P, size_t* idx) {
 int ypass(kw_plan* P, int mode, const std::vector<int>& bufs) {
   if (P->g.nd != 3 || bufs.empty()) return KW_OK;
   ColArgs a = col_args(P->g, 1, mode);
+  a.tile = P->col_tile[0];
   a.nbuf = (int)bufs.size();
   for (size_t i = 0; i < bufs.size(); ++i) a.bufs[i] = bufs[i];
   const double spec_bytes = 8.0 * P->nb * (double)P->g.rows * P->g.nh;
@@ -238,6 +241,7 @@ int ypass(kw_plan* P, int mode, const std::vector<int>& bufs) {
 
 int xpass(kw_plan* P, const std::vector<XJob>& jobs, int kind) {
   ColArgs a = col_args(P->g, 0, 2);
+  a.tile = P->col_tile[1];
   a.njob = (int)jobs.size();
   int nin = 0, last = -1;
   for (size_t i = 0; i < jobs.size(); ++i) {
@@ -650,6 +654,85 @@ int kw_plan_set_sensor(kw_plan* P, int b, int nsens, int ncorner, const int64_t*
   return sync_inst(P);
 }
 
+namespace {
+
+// Pick, per plan, between the two compiled tile widths of the FFT-only column
+// pass and of the pass with spectral jobs, by timing each on the plan's own
+// (zeroed) spectrum buffers: a forward transform of the D working spectra along y
+// (along x in 2-D), and the gradient plus the divergence pass.  The alternative
+// replaces the default only when at least 3 % faster (the timing of a few passes
+// is noisy, and the default widths were measured best at 512).  Runs once, from
+// kw_plan_init before it clears the spectra; KW_AUTOTUNE=0 keeps the defaults and
+// KW_COL_TILES="ab" forces the variants (a: FFT-only pass, b: job pass).
+int autotune_tiles(kw_plan* P) {
+  if (P->tuned) return KW_OK;
+  P->tuned = true;
+  const char* env = std::getenv("KW_AUTOTUNE");
+  if (env && std::atoi(env) == 0) return KW_OK;
+  const char* force = std::getenv("KW_COL_TILES");  // two digits, e.g. "10": FFT-only variant, job-pass variant
+  if (force && std::strlen(force) == 2) {
+    P->col_tile[0] = force[0] == '1';
+    P->col_tile[1] = force[1] == '1';
+    return KW_OK;
+  }
+  const Geom& g = P->g;
+  const int D = g.nd;
+  for (int s = 0; s < D; ++s) KW_LAUNCH(launch_fill(reinterpret_cast<float*>(P->fl.spec[s]), 2 * (int64_t)P->nb * g.nspec, 0.f, P->st));
+  int* tdev = P->fl.tdev;
+  P->fl.tdev = nullptr;  // time every instance
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = KW_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(KW_EHIP, "autotune: event creation failed");
+  std::vector<int> bufs;
+  std::vector<XJob> jobs, djobs;
+  for (int i = 0; i < D; ++i) {
+    bufs.push_back(i);
+    jobs.push_back({0, i, XK_GRAD, i});
+    djobs.push_back({i, i, XK_DIV, i});
+  }
+  for (int kind = 0; kind < 2 && rc == KW_OK; ++kind) {
+    float t_ms[2] = {0.f, 0.f};
+    for (int tile = 0; tile < 2 && rc == KW_OK; ++tile) {
+      P->col_tile[kind] = tile;
+      auto pass = [&]() -> int {
+        if (kind == 1) {
+          const int r = xpass(P, jobs, KK_XGRAD);
+          return r ? r : xpass(P, djobs, KK_XDIV);
+        }
+        if (D == 3) return ypass(P, 0, bufs);
+        ColArgs a = col_args(g, 0, 0);
+        a.tile = tile;
+        a.nbuf = D;
+        for (int i = 0; i < D; ++i) a.bufs[i] = i;
+        KW_LAUNCH(launch_col(g, P->nb, P->fl, P->ops, P->d_inst, a, g.n[0], P->st));
+        return KW_OK;
+      };
+      if ((rc = pass())) break;  // warm-up
+      if (hipEventRecord(e0, P->st) != hipSuccess) { rc = fail(KW_EHIP, "autotune: event record failed"); break; }
+      for (int r = 0; r < 3 && rc == KW_OK; ++r) rc = pass();
+      if (rc) break;
+      float ms = 0.f;
+      if (hipEventRecord(e1, P->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = fail(KW_EHIP, "autotune: timing failed"); break; }
+      t_ms[tile] = ms;
+    }
+    P->col_tile[kind] = (rc == KW_OK && t_ms[1] < 0.97f * t_ms[0]) ? 1 : 0;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  P->fl.tdev = tdev;
+  return rc;
+}
+
+}  // namespace
+
+int kw_plan_tiles(kw_plan* P, int* fft_tile, int* job_tile) {
+  if (!P || !fft_tile || !job_tile) return fail(KW_EINVAL, "NULL argument");
+  *fft_tile = P->col_tile[0];
+  *job_tile = P->col_tile[1];
+  return KW_OK;
+}
+
 int kw_plan_init(kw_plan* P) {
   if (!P) return fail(KW_EINVAL, "NULL plan");
   KW_HIP(hipSetDevice(P->dev));
@@ -669,6 +752,7 @@ int kw_plan_init(kw_plan* P) {
   if (with_p0 != 0 && with_p0 != P->nb) return fail(KW_EINVAL, "either every instance of a plan has p0 or none has");
   int rc;
   if ((rc = sync_inst(P))) return rc;
+  if ((rc = autotune_tiles(P))) return rc;
   const int64_t nf = (int64_t)P->nb * g.ntot;
   KW_LAUNCH(launch_fill(P->fl.p, nf, 0.f, P->st));
   for (int i = 0; i < g.nd; ++i) {

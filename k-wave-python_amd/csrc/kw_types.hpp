@@ -100,6 +100,7 @@ struct ColArgs {
   int64_t ostride;  // stride between consecutive "outer" indices
   int nouter;       // outer index count (3-D y-pass: nx; x-pass: ny; 2-D: 1)
   int ncols;        // columns (= nh)
+  int tile;         // 0: default tile width, 1: the alternative (chosen per plan by timing)
 };
 
 struct Fields {

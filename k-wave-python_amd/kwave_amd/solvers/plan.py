@@ -142,6 +142,12 @@ class DevicePlan:
         check(L.kw_plan_read(self._h, b, which, fptr(out), out.size))
         return out
 
+    def tiles(self):
+        """(FFT-only, spectral-job) column tile variants the plan picked by timing (0 default, 1 half width)."""
+        f, j = C.c_int(0), C.c_int(0)
+        check(_abi.lib().kw_plan_tiles(self._h, C.byref(f), C.byref(j)))
+        return f.value, j.value
+
     def step_index(self, b=0):
         return int(self.read(b, _abi.F_STEP)[0])
 

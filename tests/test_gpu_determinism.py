@@ -50,3 +50,20 @@ def test_repeat_runs_bit_identical(shape):
         for o in outs[1:]:
             assert np.array_equal(outs[0][k], o[k]), (shape, k, float(np.max(np.abs(outs[0][k] - o[k]))))
     assert np.all(np.isfinite(outs[0]["p"]))
+
+
+@pytest.mark.parametrize("shape", [(64, 80), (24, 32, 40)])
+def test_column_tile_variants_bit_identical(shape, monkeypatch):
+    """Both compiled tile widths of the column passes give the same bits (each
+    column is transformed by the same code whatever its neighbours), so the
+    per-plan tile autotuning cannot change results."""
+    from kwave_amd.solvers.kspace_solver import Simulation
+
+    kg, med, src, sen = _case(shape)
+    pml = tuple(4 for _ in shape)
+    outs = []
+    for tiles in ("00", "11"):
+        monkeypatch.setenv("KW_COL_TILES", tiles)
+        outs.append(Simulation(kg, med, copy.deepcopy(src), sen, pml_size=pml, pml_alpha=tuple(2.0 for _ in shape)).run())
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), (shape, k)
