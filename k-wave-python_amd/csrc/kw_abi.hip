@@ -147,8 +147,11 @@ struct kw_plan {
   int max_sens = 0;
   std::vector<int> nrec, nsens;
   std::vector<unsigned> rec_flags;
-  hipGraphExec_t g1 = nullptr, gk = nullptr;
+  hipGraphExec_t g1 = nullptr, gk = nullptr;  // one step, graph_k steps
+  hipGraphExec_t q1 = nullptr, qk = nullptr;  // the same once every spectral source has ended
   int graph_k = 8;
+  int64_t t_host = -1;  // steps taken since kw_plan_init (-1: not known on the host)
+  bool quiet = true;     // use the quiet step form once spectral sources have ended
   bool graphs = true;
   bool skip_gather = false;  // replayed steps of the reverse sweep do not record
   int col_tile[2] = {0, 0};  // tile variant of the FFT-only and the spectral-job column passes
@@ -163,13 +166,34 @@ struct kw_plan {
 
   ~kw_plan() {
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
-    if (g1) (void)hipGraphExecDestroy(g1);
-    if (gk) (void)hipGraphExecDestroy(gk);
+    for (hipGraphExec_t x : {g1, gk, q1, qk})
+      if (x) (void)hipGraphExecDestroy(x);
     if (st) (void)hipStreamDestroy(st);
   }
 };
 
 namespace {
+
+void drop_graphs(kw_plan* P) {
+  for (hipGraphExec_t* x : {&P->g1, &P->gk, &P->q1, &P->qk})
+    if (*x) { (void)hipGraphExecDestroy(*x); *x = nullptr; }
+}
+
+// First step at which no instance injects a source through a spectrum buffer
+// (mode CHAIN) or a pressure source through a dense pattern (mode PATTERN);
+// -1 when the plan has none.  From that step on, the step skips the source
+// transforms and the register-staged source term of the mass update, as the
+// per-step loop skips a source whose signal has ended (kspace_solver.py:96-98).
+int64_t source_end(const kw_plan* P) {
+  bool any = false;
+  int64_t end = 0;
+  for (const Inst& I : P->h_inst) {
+    const Src* ss[4] = {&I.srcp, &I.srcu[0], &I.srcu[1], &I.srcu[2]};
+    for (const Src* s : ss)
+      if (s->mode == SRC_CHAIN || (s == &I.srcp && s->mode == SRC_PATTERN)) { any = true; end = std::max<int64_t>(end, s->len); }
+  }
+  return any ? end : -1;
+}
 
 int sync_inst(kw_plan* P) {
   if (!P->inst_dirty) return KW_OK;
@@ -286,7 +310,10 @@ double zbytes_eos(const kw_plan* P) {
   return b;
 }
 
-int enqueue_step(kw_plan* P) {
+// One time step.  quiet: every spectral (CHAIN) and pattern pressure source
+// has ended, so no source spectrum is built or carried through the column
+// passes and the mass update takes its form without a staged source term.
+int enqueue_step(kw_plan* P, bool quiet = false) {
   const Geom& g = P->g;
   const int D = g.nd;
   const float2* twz = P->ops.tw[D - 1];
@@ -296,7 +323,7 @@ int enqueue_step(kw_plan* P) {
   std::vector<XJob> jobs;
   for (int i = 0; i < D; ++i) jobs.push_back({0, i, XK_GRAD, i});
   for (int i = 0; i < D; ++i)
-    if (P->chain_u[i]) {
+    if (P->chain_u[i] && !quiet) {
       KW_TLAUNCH(KK_SRC, 0.0, launch_src_build(g, P->nb, P->fl, P->d_inst, twz, i, P->st));
       ub.push_back(D + 1 + i);
       jobs.push_back({D + 1 + i, D + 1 + i, XK_SRCKAPPA, 0});
@@ -318,7 +345,8 @@ int enqueue_step(kw_plan* P) {
     yb.push_back(i);
     if (P->rec_uc[i]) { jobs.push_back({i, 2 * D + 1 + i, XK_UNSTAGGER, i}); yb.push_back(2 * D + 1 + i); }
   }
-  if (P->chain_p) {
+  const bool chain_p = P->chain_p && !quiet;
+  if (chain_p) {
     KW_TLAUNCH(KK_SRC, 0.0, launch_src_build(g, P->nb, P->fl, P->d_inst, twz, -1, P->st));
     jobs.push_back({D, D, XK_SRCKAPPA, 0});
     yb.push_back(D);
@@ -329,7 +357,7 @@ int enqueue_step(kw_plan* P) {
   for (int i = 0; i < D; ++i)
     if (P->rec_uc[i])
       KW_TLAUNCH(KK_SRC, 0.0, launch_zinv(g, P->nb, P->fl.spec[2 * D + 1 + i], g.nspec, P->fl.uc[i], g.ntot, twz, 1.f, P->st));
-  KW_TLAUNCH(KK_MASS, zbytes_mass(P), launch_mass(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st, P->chain_p));
+  KW_TLAUNCH(KK_MASS, zbytes_mass(P), launch_mass(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st, (P->chain_p || P->pattern_p) && !quiet));
   if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP)) {
     jobs.clear();
     yb.clear();
@@ -346,11 +374,12 @@ int enqueue_step(kw_plan* P) {
 }
 
 int build_graphs(kw_plan* P) {
-  for (int pass = 0; pass < 2; ++pass) {
-    const int nsteps = pass == 0 ? 1 : P->graph_k;
+  const bool quiet_too = P->quiet && source_end(P) >= 0;
+  for (int pass = 0; pass < (quiet_too ? 4 : 2); ++pass) {
+    const int nsteps = pass % 2 == 0 ? 1 : P->graph_k;
     KW_HIP(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
     int rc = KW_OK;
-    for (int s = 0; s < nsteps && rc == KW_OK; ++s) rc = enqueue_step(P);
+    for (int s = 0; s < nsteps && rc == KW_OK; ++s) rc = enqueue_step(P, pass >= 2);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(P->st, &graph);
     if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
@@ -359,8 +388,35 @@ int build_graphs(kw_plan* P) {
     e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(KW_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    (pass == 0 ? P->g1 : P->gk) = ex;
+    hipGraphExec_t* slot[4] = {&P->g1, &P->gk, &P->q1, &P->qk};
+    *slot[pass] = ex;
   }
+  return KW_OK;
+}
+
+// Enqueue nsteps steps: the quiet form once the host knows every spectral
+// source has ended, the full form otherwise (always correct: the kernels test
+// the signal length themselves).
+int launch_steps(kw_plan* P, int64_t nsteps) {
+  int rc;
+  const int64_t qend = P->quiet ? source_end(P) : -1;
+  if (P->graphs && !P->g1 && (rc = build_graphs(P))) return rc;
+  const bool can_quiet = qend >= 0 && P->t_host >= 0 && (!P->graphs || P->q1);
+  auto quiet_at = [&](int64_t s) { return can_quiet && P->t_host + s >= qend; };
+  for (int64_t s = 0; s < nsteps;) {
+    const bool q = quiet_at(s);
+    if (!P->graphs) {
+      if ((rc = enqueue_step(P, q))) return rc;
+      ++s;
+      continue;
+    }
+    // a graph_k block runs in one form: a full block that would reach the
+    // quiet steps is split into single steps up to them
+    const bool block = s + P->graph_k <= nsteps && (q || !quiet_at(s + P->graph_k - 1));
+    KW_HIP(hipGraphLaunch(block ? (q ? P->qk : P->gk) : (q ? P->q1 : P->g1), P->st));
+    s += block ? P->graph_k : 1;
+  }
+  if (P->t_host >= 0) P->t_host += nsteps;
   return KW_OK;
 }
 
@@ -406,6 +462,7 @@ int kw_plan_create(const kw_plan_desc* d, kw_plan** out) {
   P->use_sg = d->use_sg != 0;
   for (int i = 0; i < 3; ++i) P->spacing[i] = d->spacing[i];
   P->graphs = std::getenv("KW_NO_GRAPH") == nullptr;
+  P->quiet = std::getenv("KW_NO_QUIET") == nullptr;
   int rc = KW_OK;
   auto cleanup = [&](int code) { delete P; return code; };
   if (hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking) != hipSuccess) return cleanup(fail(KW_EHIP, "stream creation failed"));
@@ -581,16 +638,14 @@ int kw_plan_set_source(kw_plan* P, int b, int which, int mode, int64_t npts, con
     s.pattern = pat;
     if (which < 0 && !P->pattern_p) {
       P->pattern_p = true;
-      if (P->g1) { (void)hipGraphExecDestroy(P->g1); P->g1 = nullptr; }
-      if (P->gk) { (void)hipGraphExecDestroy(P->gk); P->gk = nullptr; }
+      drop_graphs(P);
     }
   } else {
     s.mode = SRC_CHAIN;
     s.chain_buf = which < 0 ? g.nd : g.nd + 1 + which;
     if ((rc = ensure_spec(P, s.chain_buf))) return rc;
     if (which < 0) P->chain_p = true; else P->chain_u[which] = true;
-    if (P->g1) { (void)hipGraphExecDestroy(P->g1); P->g1 = nullptr; }
-    if (P->gk) { (void)hipGraphExecDestroy(P->gk); P->gk = nullptr; }
+    drop_graphs(P);
   }
   if (which < 0) P->h_inst[b].srcp = s; else P->h_inst[b].srcu[which] = s;
   P->inst_dirty = true;
@@ -637,8 +692,7 @@ int kw_plan_set_sensor(kw_plan* P, int b, int nsens, int ncorner, const int64_t*
         if ((rc = ensure_spec(P, 2 * P->g.nd + 1 + i))) return rc;
         if ((rc = P->mem.alloc(&P->fl.uc[i], (size_t)P->nb * P->g.ntot))) return rc;
         P->rec_uc[i] = true;
-        if (P->g1) { (void)hipGraphExecDestroy(P->g1); P->g1 = nullptr; }
-        if (P->gk) { (void)hipGraphExecDestroy(P->gk); P->gk = nullptr; }
+        drop_graphs(P);
       }
     }
   }
@@ -647,8 +701,7 @@ int kw_plan_set_sensor(kw_plan* P, int b, int nsens, int ncorner, const int64_t*
   P->rec_flags[b] = rec_flags;
   if (nsens > P->max_sens) {
     P->max_sens = nsens;
-    if (P->g1) { (void)hipGraphExecDestroy(P->g1); P->g1 = nullptr; }
-    if (P->gk) { (void)hipGraphExecDestroy(P->gk); P->gk = nullptr; }
+    drop_graphs(P);
   }
   P->inst_dirty = true;
   return sync_inst(P);
@@ -736,6 +789,7 @@ int kw_plan_tiles(kw_plan* P, int* fft_tile, int* job_tile) {
 int kw_plan_init(kw_plan* P) {
   if (!P) return fail(KW_EINVAL, "NULL plan");
   KW_HIP(hipSetDevice(P->dev));
+  P->t_host = -1;
   const Geom& g = P->g;
   int with_p0 = 0;
   for (int b = 0; b < P->nb; ++b) {
@@ -797,6 +851,9 @@ int kw_plan_init(kw_plan* P) {
     KW_LAUNCH(launch_tick(P->fl, P->d_inst, P->nb, P->st));
   }
   KW_HIP(hipStreamSynchronize(P->st));
+  int t0 = 0;  // every instance starts at the same step
+  KW_HIP(hipMemcpy(&t0, P->fl.tdev, sizeof(int), hipMemcpyDeviceToHost));
+  P->t_host = t0;
   return KW_OK;
 }
 
@@ -805,15 +862,7 @@ int kw_plan_launch(kw_plan* P, int64_t nsteps) {
   KW_HIP(hipSetDevice(P->dev));
   int rc;
   if ((rc = sync_inst(P))) return rc;
-  if (P->graphs) {
-    if (!P->g1 && (rc = build_graphs(P))) return rc;
-    int64_t s = 0;
-    for (; s + P->graph_k <= nsteps; s += P->graph_k) KW_HIP(hipGraphLaunch(P->gk, P->st));
-    for (; s < nsteps; ++s) KW_HIP(hipGraphLaunch(P->g1, P->st));
-  } else {
-    for (int64_t s = 0; s < nsteps; ++s)
-      if ((rc = enqueue_step(P))) return rc;
-  }
+  if ((rc = launch_steps(P, nsteps))) return rc;
   return KW_OK;
 }
 
@@ -829,15 +878,7 @@ int kw_plan_run(kw_plan* P, int64_t nsteps) {
   KW_HIP(hipSetDevice(P->dev));
   int rc;
   if ((rc = sync_inst(P))) return rc;
-  if (P->graphs) {
-    if (!P->g1 && (rc = build_graphs(P))) return rc;
-    int64_t s = 0;
-    for (; s + P->graph_k <= nsteps; s += P->graph_k) KW_HIP(hipGraphLaunch(P->gk, P->st));
-    for (; s < nsteps; ++s) KW_HIP(hipGraphLaunch(P->g1, P->st));
-  } else {
-    for (int64_t s = 0; s < nsteps; ++s)
-      if ((rc = enqueue_step(P))) return rc;
-  }
+  if ((rc = launch_steps(P, nsteps))) return rc;
   KW_HIP(hipStreamSynchronize(P->st));
   return KW_OK;
 }
@@ -851,8 +892,10 @@ int kw_plan_run_profiled(kw_plan* P, int64_t nsteps, double* kernel_ms, double* 
   P->prof = true;
   P->ev_used = 0;
   P->recs.clear();
-  for (int64_t s = 0; s < nsteps && rc == KW_OK; ++s) rc = enqueue_step(P);
+  const int64_t qend = P->quiet ? source_end(P) : -1;
+  for (int64_t s = 0; s < nsteps && rc == KW_OK; ++s) rc = enqueue_step(P, qend >= 0 && P->t_host >= 0 && P->t_host + s >= qend);
   P->prof = false;
+  if (P->t_host >= 0) P->t_host += nsteps;
   if (rc) return rc;
   KW_HIP(hipStreamSynchronize(P->st));
   for (const auto& r : P->recs) {
@@ -1008,6 +1051,7 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   };
   // forward pass: records the traces, keeps the state before every ckpt-th step
   if ((rc = kw_plan_init(P))) return rc;
+  P->t_host = -1;  // the sweep below sets the step counter itself
   P->skip_gather = false;
   for (int t = 0; t < nt; ++t) {
     if (t % ckpt == 0 && (rc = copy_state(t / ckpt, true))) return rc;

@@ -44,8 +44,8 @@ hipError_t launch_src_build(const Geom& g, int nb, const Fields& fl, const Inst*
 hipError_t launch_momentum(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st) {
   KW_TRY4(launch_momentum, g, nb, fl, inst, tw, st);
 }
-hipError_t launch_mass(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int flags, hipStream_t st, bool chain) {
-  KW_TRY4(launch_mass, g, nb, fl, inst, tw, flags, st, chain);
+hipError_t launch_mass(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int flags, hipStream_t st, bool sreg) {
+  KW_TRY4(launch_mass, g, nb, fl, inst, tw, flags, st, sreg);
 }
 hipError_t launch_eos(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int flags, hipStream_t st) {
   KW_TRY4(launch_eos, g, nb, fl, inst, tw, flags, st);

@@ -375,8 +375,13 @@ __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MOM) void k_momentum(
 
 // ---------------------------------------------------------------------------
 // mass conservation (+ equation of state when no fractional Laplacian is needed)
+// SREG: the pressure source term is staged in registers once per element
+// (CHAIN: the inverse transform of its filtered spectrum; PATTERN: sig(t) times
+// the dense filtered pattern) and added to every split density from there.
+// The other instantiation serves plans without such a source and the steps
+// after the signal has ended.
 // ---------------------------------------------------------------------------
-template <int N, int ND, bool CHAIN>
+template <int N, int ND, bool SREG>
 __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MASS) void k_mass(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g, int flags) {
   KW_ZPASS_PROLOGUE
   const Inst& I = inst[b];
@@ -388,7 +393,8 @@ __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MASS) void k_mass(Geo
   const int64_t off = b * g.ntot;
   const Src& src = I.srcp;
   const int smode = src_mode_at(src, t);
-  const bool chain = CHAIN && smode == SRC_CHAIN;
+  const bool chain = SREG && smode == SRC_CHAIN;
+  const bool patt = SREG && smode == SRC_PATTERN;
   const float sig_t = smode == SRC_PATTERN ? gp(src.sig)[t] : 0.f;
   const bool powerlaw = flags & (FLAG_POWERLAW_ABS | FLAG_POWERLAW_DISP);
   float ph[ND][2];
@@ -399,7 +405,7 @@ __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MASS) void k_mass(Geo
   // of the previous step, kept in rhot), and the running sums of div u and of
   // the new densities
   float2 qv[C::E], divu[C::E], rt[C::E];
-  float2 ch[CHAIN ? C::E : 1];
+  float2 ch[SREG ? C::E : 1];
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
     const int64_t i0 = iA + m * C::TPL;
@@ -412,8 +418,14 @@ __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MASS) void k_mass(Geo
     divu[m] = make_float2(0.f, 0.f);
     rt[m] = make_float2(0.f, 0.f);
   }
-  if constexpr (CHAIN) {
-    if (chain) inv_to_regs<N>(lds, fl.spec[src.chain_buf] + b * g.nspec, tw, row0, rows, g.nhp, tid, ch);
+  if constexpr (SREG) {
+    if (chain) {
+      inv_to_regs<N>(lds, fl.spec[src.chain_buf] + b * g.nspec, tw, row0, rows, g.nhp, tid, ch);
+    } else if (patt) {
+      const auto pat = gp(src.pattern);
+#pragma unroll
+      for (int m = 0; m < C::E; ++m) ch[m] = make_float2(sig_t * pat[iA + m * C::TPL], sig_t * pat[iA + N + m * C::TPL]);
+    }
   }
 #pragma unroll
   for (int i = 0; i < ND; ++i) {
@@ -442,9 +454,11 @@ __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MASS) void k_mass(Geo
         if constexpr (PF) rold = h ? ro[m].y : ro[m].x;
         else rold = rho[idx];
         float rv = s * (s * rold - q * dv);
-        float chv = 0.f;
-        if constexpr (CHAIN) chv = chain ? (h ? ch[m].y : ch[m].x) : 0.f;
-        rv = src_inline(src, smode, rv, idx, src.fac[i], sig_t, chv);
+        if constexpr (SREG) {
+          if (chain || patt) rv = rv + src.fac[i] * (h ? ch[m].y : ch[m].x);
+        } else {
+          rv = src_inline(src, smode, rv, idx, src.fac[i], sig_t, 0.f);
+        }
         res[h] = rv;
       }
       if (inside) {

@@ -67,3 +67,68 @@ def test_column_tile_variants_bit_identical(shape, monkeypatch):
         outs.append(Simulation(kg, med, copy.deepcopy(src), sen, pml_size=pml, pml_alpha=tuple(2.0 for _ in shape)).run())
     for k in outs[0]:
         assert np.array_equal(outs[0][k], outs[1][k]), (shape, k)
+
+
+def _source_case(shape, per_point, nt=83, slen=29):
+    """Multi-row additive p and u sources whose signals end before nt: one
+    signal row (the dense-pattern form) or one row per point (the spectral
+    form)."""
+    from kwave_amd.data import Vector
+    from kwave_amd.kgrid import kWaveGrid
+    from kwave_amd.kmedium import kWaveMedium
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.ksource import kSource
+
+    rng = np.random.default_rng(5)
+    dx = 1e-4
+    kg = kWaveGrid(Vector(list(shape)), Vector([dx] * len(shape)))
+    kg.setTime(nt, 0.3 * dx / 1700.0)
+    med = kWaveMedium(sound_speed=1500.0 + 200.0 * rng.random(shape), density=1000.0, alpha_coeff=0.5, alpha_power=1.3)
+    src = kSource()
+    pm = np.zeros(shape, dtype=bool)
+    pm[(slice(shape[0] // 4, shape[0] // 4 + 6),) + tuple(slice(s // 2 - 2, s // 2 + 2) for s in shape[1:])] = True
+    src.p_mask = pm
+    um = np.zeros(shape, dtype=bool)
+    um[(slice(shape[0] // 2, shape[0] // 2 + 3),) + tuple(slice(s // 3, s // 3 + 3) for s in shape[1:])] = True
+    src.u_mask = um
+    np_, nu = (int(pm.sum()), int(um.sum())) if per_point else (1, 1)
+    src.p = 1e4 * np.sin(np.arange(slen)[None, :] * 0.4 + np.arange(np_)[:, None] * 0.1)
+    src.ux = 0.05 * np.cos(np.arange(slen - 7)[None, :] * 0.3 + np.arange(nu)[:, None] * 0.2)
+    sm = np.zeros(shape, dtype=bool)
+    sm[(slice(None),) * (len(shape) - 1) + (shape[-1] // 3,)] = True
+    sen = kSensor(sm)
+    sen.record = ["p", "u", "p_final"]
+    return kg, med, src, sen
+
+
+@pytest.mark.parametrize("per_point", [False, True], ids=["pattern", "spectral"])
+@pytest.mark.parametrize("shape", [(64, 80), (24, 32, 40)])
+def test_quiet_steps_after_sources_end_bit_identical(shape, per_point, monkeypatch):
+    """Once every filtered source has ended the step leaves out the source
+    transforms and the staged source term; results equal the full step's bit
+    for bit, through run() (graph blocks split at the switch), step() and the
+    graph-free path."""
+    from kwave_amd.solvers.kspace_solver import Simulation
+
+    kg, med, src, sen = _source_case(shape, per_point)
+    pml = tuple(4 for _ in shape)
+    alpha = tuple(2.0 for _ in shape)
+
+    def run(env, stepwise=False):
+        for k in ("KW_NO_QUIET", "KW_NO_GRAPH"):
+            monkeypatch.delenv(k, raising=False)
+        for k in env:
+            monkeypatch.setenv(k, "1")
+        sim = Simulation(kg, med, copy.deepcopy(src), sen, pml_size=pml, pml_alpha=alpha)
+        if not stepwise:
+            return sim.run()
+        sim.setup()
+        while sim.t < sim.Nt:
+            sim.step()
+        return sim.run()
+
+    ref = run(("KW_NO_QUIET",))
+    assert np.abs(ref["p"]).max() > 0
+    for out in (run(()), run((), stepwise=True), run(("KW_NO_GRAPH",))):
+        for k in ref:
+            assert np.array_equal(ref[k], out[k]), (shape, k)
