@@ -15,8 +15,11 @@ Network (runs at 64^2, the c map averaged over 2 x 2 cells):
            per-sensor mean trace.
 
 The spectral contraction of every layer - forward, input gradient and weight
-gradient - runs in the library's BF16 MFMA kernel (kw_mode_gemm_bf16, FP32
-accumulation); FFTs, pointwise layers and the optimiser are PyTorch.
+gradient - runs in the library's BF16 MFMA kernels (kw_mode_contract_bf16,
+kw_mode_gemm_bf16, FP32 accumulation).  Only 2 md x md of the 64 x 33 Fourier
+modes are kept, so the truncated transforms are dense matrix products (two per
+direction, one per axis) rather than full FFTs; pointwise layers and the
+optimiser are PyTorch.
 """
 from __future__ import annotations
 
@@ -166,6 +169,45 @@ def spectral_contract(x2, w2):
 
 
 # ---------------------------------------------------------------------------
+# truncated 2-D DFT as matrix products
+# ---------------------------------------------------------------------------
+def dft_mats(n, md):
+    """Real matrices of the truncated forward and inverse 2-D DFT of an n x n field.
+
+    Retained modes: kx in [0, md) and [n - md, n) (row index j of the compact
+    spectrum, 2 md rows), ky in [0, md).  Complex values are interleaved
+    (re, im) along the axis they sit on.
+
+      forward   B = Mx v       (4 md x n: rows 2j, 2j+1 = Re, Im of sum_x v e^{-i kx_j x})
+                X = B' My      (B' = B viewed 2 md x 2n, row j = [Re B_j | Im B_j];
+                                X: 2 md x 2 md, columns 2k, 2k+1 = Re, Im at ky = k)
+      inverse   C = Y Ny       (2 md x 2n, row j = [Re C_j | Im C_j],
+                                C_j(y) = sum_k w_k Y_jk e^{+i k y}, w_0 = 1, w_k = 2)
+                v = Nx C'      (C' = C viewed 4 md x n; v = Re sum_j C_j e^{+i kx_j x} / n^2)
+
+    X equals torch.fft.rfft2(v) at the retained modes and the inverse equals
+    torch.fft.irfft2 of the spectrum that is zero elsewhere (the Nyquist column
+    ky = n/2 is not retained).
+    """
+    kx = np.concatenate([np.arange(md), np.arange(n - md, n)])
+    x = np.arange(n)
+    ph = 2 * np.pi * np.outer(kx, x) / n                                    # (2 md, n)
+    mx = np.zeros((4 * md, n))
+    mx[0::2], mx[1::2] = np.cos(ph), -np.sin(ph)
+    th = 2 * np.pi * np.outer(x, np.arange(md)) / n                         # (n, md)
+    my = np.zeros((2 * n, 2 * md))
+    my[:n, 0::2], my[n:, 0::2] = np.cos(th), np.sin(th)
+    my[:n, 1::2], my[n:, 1::2] = -np.sin(th), np.cos(th)
+    w = np.where(np.arange(md) == 0, 1.0, 2.0)[:, None]
+    ny_ = np.zeros((2 * md, 2 * n))
+    ny_[0::2, :n], ny_[1::2, :n] = w * np.cos(th.T), -w * np.sin(th.T)
+    ny_[0::2, n:], ny_[1::2, n:] = w * np.sin(th.T), w * np.cos(th.T)
+    nx_ = np.zeros((n, 4 * md))
+    nx_[:, 0::2], nx_[:, 1::2] = np.cos(ph.T) / n ** 2, -np.sin(ph.T) / n ** 2
+    return [a.astype(np.float32) for a in (mx, my, ny_, nx_)]
+
+
+# ---------------------------------------------------------------------------
 # the network
 # ---------------------------------------------------------------------------
 def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
@@ -190,7 +232,8 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
             scale = 1.0 / (c * c)
             self.wr = nn.Parameter(scale * torch.randn(2 * modes * modes, c, c))
             self.wi = nn.Parameter(scale * torch.randn(2 * modes * modes, c, c))
-            self.register_buffer("kx", torch.cat([torch.arange(modes), torch.arange(n - modes, n)]))
+            for name, a in zip(("mx", "my", "ny", "nx"), dft_mats(n, modes)):
+                self.register_buffer(name, torch.as_tensor(a))
 
         def weights(self):
             c = self.c
@@ -198,9 +241,13 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
             return w2.reshape(-1, 2 * c, 2 * c)
 
         def forward(self, v):
-            n = v.shape[-1]
-            ft = torch.fft.rfft2(v.float())                                      # (b, c, n, n/2+1)
-            return torch.fft.irfft2(spectral_modes(ft, self.weights(), self.modes), s=(n, n))
+            b, c, n, _ = v.shape
+            md = self.modes
+            bx = torch.matmul(self.mx, v).reshape(b, c, 2 * md, 2 * n)
+            xm = torch.matmul(bx, self.my).float().reshape(b, c, 2 * md, md, 2)  # retained modes
+            ym = spectral_modes(torch.view_as_complex(xm), self.weights(), md)
+            cy = torch.matmul(torch.view_as_real(ym).reshape(b, c, 2 * md, 2 * md), self.ny)
+            return torch.matmul(self.nx, cy.reshape(b, c, 4 * md, n))
 
     class FNO(nn.Module):
         def __init__(self, n_sens, nt, cols_a, cols_b, cols_f):
@@ -219,7 +266,7 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
             nn.init.zeros_(self.head_b.weight)
             nn.init.zeros_(self.head_f.weight)
             self.mean = nn.Parameter(torch.zeros(n_sens, nt))
-            self.register_buffer("cols_ab", torch.as_tensor(np.concatenate([cols_a, cols_b]), dtype=torch.long))
+            self.register_buffer("cols_ab", torch.as_tensor(np.stack([cols_a, cols_b], 1).reshape(-1), dtype=torch.long))
             self.register_buffer("cols_f", torch.as_tensor(cols_f, dtype=torch.long))
             g = torch.linspace(0, 1, N_COARSE)
             self.register_buffer("grid", torch.stack(torch.meshgrid(g, g, indexing="ij"))[None])
@@ -237,9 +284,12 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
                 v = self.spec[i](v) + w.reshape(b, wd, n, n)
                 if i < self.layers - 1:
                     v = F.gelu(v)
-            sel = v.index_select(3, self.cols_ab).permute(0, 3, 1, 2).reshape(b, 2, self.n_sens, wd * n)
+            # per sensor s the feature rows [column a_s | column b_s] (2 x wd x n), one product for both heads
+            sel = v.permute(0, 3, 1, 2).index_select(1, self.cols_ab).reshape(b * self.n_sens, 2 * wd * n)
+            wab = torch.cat([self.head_a.weight, self.head_b.weight], 1)
+            ha = F.linear(sel, wab, self.head_a.bias).reshape(b, self.n_sens, -1)
             ff = cn.index_select(2, self.cols_f).permute(0, 2, 1)                # (b, n_sens, 128)
-            return self.mean[None] + self.head_a(sel[:, 0]) + self.head_b(sel[:, 1]) + self.head_f(ff)
+            return self.mean[None] + ha + self.head_f(ff)
 
     kg, med, src, sen, pml, alpha = workloads.surrogate_2d(0)
     ys = np.where(np.asarray(sen.mask).any(axis=0))[0]
