@@ -189,6 +189,16 @@ int kw_mode_gemm_bf16(int nmodes, int R, int N, int K, const void* A, const void
 int kw_mode_contract_bf16(int B, int C, int md, int NX, int NY, const void* X, const int64_t xstride[4], const void* Bt, void* Y,
                           const int64_t ystride[4], void* stream);
 
+/* Pointwise part of a surrogate layer for inference, fused into one pass:
+ * out[b][o][p] = act(s[b][o][p] + pb[o] + sum_c pw[o][c] v[b][c][p]), with v (the
+ * layer input), s (its spectral-conv output) and out bf16 device arrays
+ * [B][C][P] (contiguous, 8-byte aligned), pw [C][C] and pb [C] fp32 device
+ * arrays, act the exact GELU when gelu != 0 else the identity.  C is 16 or 32,
+ * P a multiple of 4; FP32 accumulation, one bf16 rounding of the result.
+ * Enqueued on `stream`.  No counterpart in the reference (it has no surrogate). */
+int kw_fno_pointwise_bf16(int B, int C, int64_t P, const void* v, const void* s, const float* pw, const float* pb, void* out, int gelu,
+                          void* stream);
+
 /* Spectral operator f -> real(ifftn(op * fftn(f))) on the device, the
  * reference's Simulation._diff (kspace_solver.py:781-796), for a single field.
  * kind: 0 identity, 1 gradient (op_grad), 2 divergence (op_div),

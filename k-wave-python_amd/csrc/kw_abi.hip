@@ -1106,6 +1106,19 @@ int kw_mode_gemm_bf16(int nmodes, int R, int N, int K, const void* A, const void
   return KW_OK;
 }
 
+int kw_fno_pointwise_bf16(int B, int C, int64_t P, const void* v, const void* s, const float* pw, const float* pb, void* out, int gelu,
+                          void* stream) {
+  if (!v || !s || !pw || !pb || !out || B < 1 || P < 4) return fail(KW_EINVAL, "FNO pointwise: NULL pointer or empty shape");
+  if (C != 16 && C != 32) return fail(KW_EUNSUPPORTED, "FNO pointwise: channel count must be 16 or 32");
+  if (P % 4) return fail(KW_EINVAL, "FNO pointwise: positions per channel must be a multiple of 4");
+  if (B > 65535) return fail(KW_EINVAL, "FNO pointwise: at most 65535 samples per call");
+  if (((uintptr_t)v | (uintptr_t)s | (uintptr_t)out) & 7) return fail(KW_EINVAL, "FNO pointwise: bf16 arrays must be 8-byte aligned");
+  hipError_t e = launch_fno_pointwise(B, C, P, static_cast<const uint16_t*>(v), static_cast<const uint16_t*>(s), pw, pb,
+                                      static_cast<uint16_t*>(out), gelu ? 1 : 0, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(KW_EHIP, std::string("FNO pointwise launch: ") + hipGetErrorString(e));
+  return KW_OK;
+}
+
 int kw_mode_contract_bf16(int B, int C, int md, int NX, int NY, const void* X, const int64_t xstride[4], const void* Bt, void* Y,
                           const int64_t ystride[4], void* stream) {
   if (!X || !Bt || !Y || !xstride || !ystride || B < 1 || md < 1) return fail(KW_EINVAL, "mode contraction: NULL pointer or empty shape");

@@ -95,6 +95,65 @@ hipError_t launch_mode_contract(int B, int K, int md, int NX, const float2* X, c
   return hipGetLastError();
 }
 
+// Pointwise part of a layer, fused for inference:
+//   out[b][o][p] = act(s[b][o][p] + pb[o] + sum_c pw[o][c] v[b][c][p])
+// v, s, out bf16 [B][C][P]; pw [C][C], pb [C] fp32; act = exact GELU
+// 0.5 x (1 + erf(x / sqrt 2)) or identity.  Each thread owns four consecutive
+// positions of one sample (8-byte loads per channel), accumulates in FP32 and
+// rounds the result to bf16 once.  The weights are uniform across the
+// workgroup and are read through scalar loads.
+template <int C>
+__global__ __launch_bounds__(256) void k_fno_pointwise(int64_t P4, const uint16_t* __restrict__ v, const uint16_t* __restrict__ s,
+                                                       const float* __restrict__ pw, const float* __restrict__ pb,
+                                                       uint16_t* __restrict__ out, int gelu) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (t >= P4) return;
+  const int64_t P = 4 * P4;
+  const int64_t base = (int64_t)b * C * P + 4 * t;
+  float x[C][4];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const ushort4 q = *reinterpret_cast<const ushort4*>(v + base + (int64_t)c * P);
+    x[c][0] = __uint_as_float((unsigned)q.x << 16);
+    x[c][1] = __uint_as_float((unsigned)q.y << 16);
+    x[c][2] = __uint_as_float((unsigned)q.z << 16);
+    x[c][3] = __uint_as_float((unsigned)q.w << 16);
+  }
+#pragma unroll 4
+  for (int o = 0; o < C; ++o) {
+    const ushort4 q = *reinterpret_cast<const ushort4*>(s + base + (int64_t)o * P);
+    float y[4] = {__uint_as_float((unsigned)q.x << 16), __uint_as_float((unsigned)q.y << 16),
+                  __uint_as_float((unsigned)q.z << 16), __uint_as_float((unsigned)q.w << 16)};
+    float w[4] = {pb[o], pb[o], pb[o], pb[o]};
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const float a = pw[o * C + c];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) w[j] = fmaf(a, x[c][j], w[j]);
+    }
+    short r[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float z = y[j] + w[j];
+      if (gelu) z = 0.5f * z * (1.f + erff(z * 0.70710678118654752f));
+      r[j] = bf16_rne(z);
+    }
+    *reinterpret_cast<ushort4*>(out + base + (int64_t)o * P) =
+        make_ushort4((unsigned short)r[0], (unsigned short)r[1], (unsigned short)r[2], (unsigned short)r[3]);
+  }
+}
+
+hipError_t launch_fno_pointwise(int B, int C, int64_t P, const uint16_t* v, const uint16_t* s, const float* pw, const float* pb,
+                                uint16_t* out, int gelu, hipStream_t st) {
+  const int64_t P4 = P / 4;
+  const dim3 grid((unsigned)((P4 + 255) / 256), (unsigned)B);
+  if (C == 16) hipLaunchKernelGGL(k_fno_pointwise<16>, grid, dim3(256), 0, st, P4, v, s, pw, pb, out, gelu);
+  else if (C == 32) hipLaunchKernelGGL(k_fno_pointwise<32>, grid, dim3(256), 0, st, P4, v, s, pw, pb, out, gelu);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 hipError_t launch_mode_gemm(int M, int R, int N, int K, const uint16_t* A, const uint16_t* Bt, float* C, hipStream_t st) {
   hipLaunchKernelGGL(k_mode_gemm, dim3(N / 32, R / 32, M), dim3(64), 0, st, R, N, K, A, Bt, C);
   return hipGetLastError();

@@ -230,7 +230,7 @@ __device__ __forceinline__ void set_lane(float2 (&a)[E], int m, int h, float v) 
 #define KW_WPE_COL 1
 #endif
 #ifndef KW_WPE_COLOPS
-#define KW_WPE_COLOPS 1
+#define KW_WPE_COLOPS 4  // the prefetch of the next job input would take it to 130 VGPRs, 3 waves
 #endif
 #define KW_WPE(N, W) __attribute__((amdgpu_waves_per_eu((FftShape<N>::E <= 8 ? (W) : 1))))
 
@@ -804,14 +804,39 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COL
     const float kx = gp(ops.k[0])[j0 + m * C::TPL];
     kap_m[m] = kap > 0.f ? sinc_k(kap * sqrtf(fmaf(kx, kx, kyz2))) : 1.f;
   }
-  int cur_in = -1;
+  // The next distinct input is loaded while the current one is transformed and
+  // its jobs stored: issued after those stores, its loads would also wait for
+  // them (the vector-memory counter drains in order).  Not when one of the jobs
+  // before it writes that buffer, and only for short register arrays (the
+  // mixed-radix ones would lose occupancy).
+  constexpr bool PRE = C::E <= 8;
+  float2 nx[PRE ? C::E : 1];
+  int cur_in = -1, pre_in = -1;
   for (int j = 0; j < a.njob; ++j) {
     const XJob jb = a.job[j];
     if (jb.in != cur_in) {
       cur_in = jb.in;
-      const auto src = gp(fl.spec[jb.in]);
+      if (PRE && pre_in == cur_in) {
 #pragma unroll
-      for (int m = 0; m < C::E; ++m) st[m] = valid ? ld2(src, base + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
+        for (int m = 0; m < C::E; ++m) st[m] = nx[PRE ? m : 0];
+      } else {
+        const auto src = gp(fl.spec[jb.in]);
+#pragma unroll
+        for (int m = 0; m < C::E; ++m) st[m] = valid ? ld2(src, base + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
+      }
+      pre_in = -1;
+      int k = j + 1;
+      while (k < a.njob && a.job[k].in == cur_in) ++k;
+      if (PRE && k < a.njob) {
+        bool clobbered = false;
+        for (int i = j; i < k; ++i) clobbered |= a.job[i].out == a.job[k].in;
+        if (!clobbered) {
+          pre_in = a.job[k].in;
+          const auto nsrc = gp(fl.spec[pre_in]);
+#pragma unroll
+          for (int m = 0; m < C::E; ++m) nx[PRE ? m : 0] = valid ? ld2(nsrc, base + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
+        }
+      }
       fft_reg<N, -1, typename C::Idx, EM>(st, lds, tw, c, j0, idx);
     }
     // where this job's operator values live: an x-table slot, the per-column

@@ -19,7 +19,8 @@ gradient - runs in the library's BF16 MFMA kernels (kw_mode_contract_bf16,
 kw_mode_gemm_bf16, FP32 accumulation).  Only 2 md x md of the 64 x 33 Fourier
 modes are kept, so the truncated transforms are dense matrix products (two per
 direction, one per axis) rather than full FFTs; pointwise layers and the
-optimiser are PyTorch.
+optimiser are PyTorch; at inference each layer's pointwise part (channel mix,
+bias, residual sum, GELU) is one fused kernel (kw_fno_pointwise_bf16).
 """
 from __future__ import annotations
 
@@ -105,6 +106,29 @@ def mode_contract(x, bt, md):
     _abi.check(_abi.lib().kw_mode_contract_bf16(b, c, md, nx, ny, C.c_void_p(x.data_ptr()), xs, C.c_void_p(bt.data_ptr()),
                                                 C.c_void_p(y.data_ptr()), ys, C.c_void_p(stream)))
     return y
+
+
+def fno_pointwise(v, s, pw, pb, gelu):
+    """act(s + pb + pw v) per sample and position in one pass (inference; kw_fno_pointwise_bf16).
+
+    v, s: bfloat16 CUDA tensors (B, C, ...) of the same shape; pw (C, C), pb (C,) or
+    (C, 1) weights; returns bfloat16 of v's shape.
+    """
+    import torch
+
+    if v.dtype != torch.bfloat16 or s.dtype != torch.bfloat16 or not v.is_cuda or v.shape != s.shape:
+        raise ValueError("fno_pointwise takes two bfloat16 device tensors of one shape")
+    v, s = v.contiguous(), s.contiguous()
+    b, c = v.shape[:2]
+    p = v[0, 0].numel()
+    pw = pw.detach().float().contiguous()
+    pb = pb.detach().float().reshape(-1).contiguous()
+    out = torch.empty_like(v)
+    stream = torch.cuda.current_stream(v.device).cuda_stream
+    _abi.check(_abi.lib().kw_fno_pointwise_bf16(b, c, p, C.c_void_p(v.data_ptr()), C.c_void_p(s.data_ptr()),
+                                                C.c_void_p(pw.data_ptr()), C.c_void_p(pb.data_ptr()),
+                                                C.c_void_p(out.data_ptr()), int(bool(gelu)), C.c_void_p(stream)))
+    return out
 
 
 def gather_modes(x, md):
@@ -279,7 +303,14 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
             x = F.avg_pool2d(cn[:, None], 2)                                     # (b, 1, n, n)
             v = x * self.lift_c[None, :, None, None] + self.lift_g(self.grid)    # lift of (c, x, y)
             wd = self.width
+            # inference under bf16 autocast: each layer's pointwise part is one fused kernel
+            fused = not torch.is_grad_enabled() and v.is_cuda and torch.is_autocast_enabled()
+            if fused:
+                v = v.to(torch.bfloat16)
             for i in range(self.layers):
+                if fused:
+                    v = fno_pointwise(v, self.spec[i](v), self.pw[i], self.pb[i], i < self.layers - 1)
+                    continue
                 w = torch.matmul(self.pw[i], v.reshape(b, wd, n * n)) + self.pb[i]
                 v = self.spec[i](v) + w.reshape(b, wd, n, n)
                 if i < self.layers - 1:

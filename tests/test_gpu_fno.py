@@ -121,3 +121,46 @@ def test_fused_mode_contraction_matches_gather_path():
     assert torch.equal(fused[1], ft.grad)
     for a, bb in zip(fused[2:], (wr.grad, wi.grad)):
         assert torch.allclose(a, bb, rtol=0, atol=1e-6 * float(bb.abs().max())), float((a - bb).abs().max())
+
+
+@pytest.mark.parametrize("c,p", [(16, 4096), (32, 12), (16, 4 * 257)])
+@pytest.mark.parametrize("gelu", [True, False])
+def test_fused_pointwise_matches_fp32_reference(c, p, gelu):
+    """kw_fno_pointwise_bf16 against the plain PyTorch FP32 form of the same op on the
+    same bf16 inputs; the only difference allowed is the final bf16 rounding."""
+    import torch
+    import torch.nn.functional as F
+
+    from kwave_amd.surrogate import fno_pointwise
+
+    g = torch.Generator().manual_seed(c + p)
+    b = 5
+    v = torch.randn(b, c, p, generator=g).to("cuda", torch.bfloat16)
+    s = torch.randn(b, c, p, generator=g).to("cuda", torch.bfloat16)
+    pw = (torch.randn(c, c, generator=g) / c ** 0.5).cuda()
+    pb = torch.randn(c, 1, generator=g).cuda()
+    out = fno_pointwise(v, s, pw, pb, gelu).float()
+    ref = s.float() + pb + torch.matmul(pw, v.float())
+    if gelu:
+        ref = F.gelu(ref)
+    err = (out - ref).abs()
+    # one bf16 rounding: relative 2^-8 of the value
+    assert float((err - 2.0 ** -8 * ref.abs()).max()) <= 1e-6, float(err.max())
+    # rounding to nearest errs by a quarter unit in the last place on average
+    assert float(err.mean()) < 2.0 ** -8 * float(ref.abs().mean())
+
+
+def test_fused_inference_matches_unfused_forward():
+    """The fused inference forward of the surrogate network agrees with its unfused
+    (autograd) form under the same bf16 autocast to within bf16 rounding."""
+    import torch
+
+    from kwave_amd import surrogate, workloads
+
+    model = surrogate.build_model().eval()
+    x = torch.as_tensor(np.stack([workloads.surrogate_c_map(5_000_000 + i) for i in range(8)]).astype(np.float32)).cuda()
+    with torch.autocast("cuda", dtype=torch.bfloat16):
+        with torch.no_grad():
+            fused = model(x).float()
+        unfused = model(x).float().detach()
+    assert _rel(fused.cpu().numpy(), unfused.cpu().numpy()) < 1e-2
