@@ -148,10 +148,10 @@ struct kw_plan {
   std::vector<int> nrec, nsens;
   std::vector<unsigned> rec_flags;
   hipGraphExec_t g1 = nullptr, gk = nullptr;  // one step, graph_k steps
-  hipGraphExec_t q1 = nullptr, qk = nullptr;  // the same once every spectral source has ended
+  hipGraphExec_t q1 = nullptr, qk = nullptr;  // the lean forms, once every filtered source has ended
   int graph_k = 8;
   int64_t t_host = -1;  // steps taken since kw_plan_init (-1: not known on the host)
-  bool quiet = true;     // use the quiet step form once spectral sources have ended
+  bool lean = true;      // use the lean step form once filtered sources have ended
   bool graphs = true;
   bool skip_gather = false;  // replayed steps of the reverse sweep do not record
   int col_tile[2] = {0, 0};  // tile variant of the FFT-only and the spectral-job column passes
@@ -310,10 +310,10 @@ double zbytes_eos(const kw_plan* P) {
   return b;
 }
 
-// One time step.  quiet: every spectral (CHAIN) and pattern pressure source
+// One time step.  lean: every spectral (CHAIN) and pattern pressure source
 // has ended, so no source spectrum is built or carried through the column
 // passes and the mass update takes its form without a staged source term.
-int enqueue_step(kw_plan* P, bool quiet = false) {
+int enqueue_step(kw_plan* P, bool lean = false) {
   const Geom& g = P->g;
   const int D = g.nd;
   const float2* twz = P->ops.tw[D - 1];
@@ -323,7 +323,7 @@ int enqueue_step(kw_plan* P, bool quiet = false) {
   std::vector<XJob> jobs;
   for (int i = 0; i < D; ++i) jobs.push_back({0, i, XK_GRAD, i});
   for (int i = 0; i < D; ++i)
-    if (P->chain_u[i] && !quiet) {
+    if (P->chain_u[i] && !lean) {
       KW_TLAUNCH(KK_SRC, 0.0, launch_src_build(g, P->nb, P->fl, P->d_inst, twz, i, P->st));
       ub.push_back(D + 1 + i);
       jobs.push_back({D + 1 + i, D + 1 + i, XK_SRCKAPPA, 0});
@@ -345,7 +345,7 @@ int enqueue_step(kw_plan* P, bool quiet = false) {
     yb.push_back(i);
     if (P->rec_uc[i]) { jobs.push_back({i, 2 * D + 1 + i, XK_UNSTAGGER, i}); yb.push_back(2 * D + 1 + i); }
   }
-  const bool chain_p = P->chain_p && !quiet;
+  const bool chain_p = P->chain_p && !lean;
   if (chain_p) {
     KW_TLAUNCH(KK_SRC, 0.0, launch_src_build(g, P->nb, P->fl, P->d_inst, twz, -1, P->st));
     jobs.push_back({D, D, XK_SRCKAPPA, 0});
@@ -357,7 +357,7 @@ int enqueue_step(kw_plan* P, bool quiet = false) {
   for (int i = 0; i < D; ++i)
     if (P->rec_uc[i])
       KW_TLAUNCH(KK_SRC, 0.0, launch_zinv(g, P->nb, P->fl.spec[2 * D + 1 + i], g.nspec, P->fl.uc[i], g.ntot, twz, 1.f, P->st));
-  KW_TLAUNCH(KK_MASS, zbytes_mass(P), launch_mass(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st, (P->chain_p || P->pattern_p) && !quiet));
+  KW_TLAUNCH(KK_MASS, zbytes_mass(P), launch_mass(g, P->nb, P->fl, P->d_inst, twz, P->flags, P->st, (P->chain_p || P->pattern_p) && !lean));
   if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP)) {
     jobs.clear();
     yb.clear();
@@ -374,8 +374,8 @@ int enqueue_step(kw_plan* P, bool quiet = false) {
 }
 
 int build_graphs(kw_plan* P) {
-  const bool quiet_too = P->quiet && source_end(P) >= 0;
-  for (int pass = 0; pass < (quiet_too ? 4 : 2); ++pass) {
+  const bool lean_too = P->lean && source_end(P) >= 0;
+  for (int pass = 0; pass < (lean_too ? 4 : 2); ++pass) {
     const int nsteps = pass % 2 == 0 ? 1 : P->graph_k;
     KW_HIP(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
     int rc = KW_OK;
@@ -394,25 +394,25 @@ int build_graphs(kw_plan* P) {
   return KW_OK;
 }
 
-// Enqueue nsteps steps: the quiet form once the host knows every spectral
+// Enqueue nsteps steps: the lean form once the host knows every spectral
 // source has ended, the full form otherwise (always correct: the kernels test
 // the signal length themselves).
 int launch_steps(kw_plan* P, int64_t nsteps) {
   int rc;
-  const int64_t qend = P->quiet ? source_end(P) : -1;
+  const int64_t qend = P->lean ? source_end(P) : -1;
   if (P->graphs && !P->g1 && (rc = build_graphs(P))) return rc;
-  const bool can_quiet = qend >= 0 && P->t_host >= 0 && (!P->graphs || P->q1);
-  auto quiet_at = [&](int64_t s) { return can_quiet && P->t_host + s >= qend; };
+  const bool can_lean = qend >= 0 && P->t_host >= 0 && (!P->graphs || P->q1);
+  auto lean_at = [&](int64_t s) { return can_lean && P->t_host + s >= qend; };
   for (int64_t s = 0; s < nsteps;) {
-    const bool q = quiet_at(s);
+    const bool q = lean_at(s);
     if (!P->graphs) {
       if ((rc = enqueue_step(P, q))) return rc;
       ++s;
       continue;
     }
     // a graph_k block runs in one form: a full block that would reach the
-    // quiet steps is split into single steps up to them
-    const bool block = s + P->graph_k <= nsteps && (q || !quiet_at(s + P->graph_k - 1));
+    // lean steps is split into single steps up to them
+    const bool block = s + P->graph_k <= nsteps && (q || !lean_at(s + P->graph_k - 1));
     KW_HIP(hipGraphLaunch(block ? (q ? P->qk : P->gk) : (q ? P->q1 : P->g1), P->st));
     s += block ? P->graph_k : 1;
   }
@@ -462,7 +462,7 @@ int kw_plan_create(const kw_plan_desc* d, kw_plan** out) {
   P->use_sg = d->use_sg != 0;
   for (int i = 0; i < 3; ++i) P->spacing[i] = d->spacing[i];
   P->graphs = std::getenv("KW_NO_GRAPH") == nullptr;
-  P->quiet = std::getenv("KW_NO_QUIET") == nullptr;
+  P->lean = std::getenv("KW_NO_LEAN") == nullptr;
   int rc = KW_OK;
   auto cleanup = [&](int code) { delete P; return code; };
   if (hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking) != hipSuccess) return cleanup(fail(KW_EHIP, "stream creation failed"));
@@ -892,7 +892,7 @@ int kw_plan_run_profiled(kw_plan* P, int64_t nsteps, double* kernel_ms, double* 
   P->prof = true;
   P->ev_used = 0;
   P->recs.clear();
-  const int64_t qend = P->quiet ? source_end(P) : -1;
+  const int64_t qend = P->lean ? source_end(P) : -1;
   for (int64_t s = 0; s < nsteps && rc == KW_OK; ++s) rc = enqueue_step(P, qend >= 0 && P->t_host >= 0 && P->t_host + s >= qend);
   P->prof = false;
   if (P->t_host >= 0) P->t_host += nsteps;

@@ -103,7 +103,7 @@ def _source_case(shape, per_point, nt=83, slen=29):
 
 @pytest.mark.parametrize("per_point", [False, True], ids=["pattern", "spectral"])
 @pytest.mark.parametrize("shape", [(64, 80), (24, 32, 40)])
-def test_quiet_steps_after_sources_end_bit_identical(shape, per_point, monkeypatch):
+def test_lean_steps_after_sources_end_bit_identical(shape, per_point, monkeypatch):
     """Once every filtered source has ended the step leaves out the source
     transforms and the staged source term; results equal the full step's bit
     for bit, through run() (graph blocks split at the switch), step() and the
@@ -115,7 +115,7 @@ def test_quiet_steps_after_sources_end_bit_identical(shape, per_point, monkeypat
     alpha = tuple(2.0 for _ in shape)
 
     def run(env, stepwise=False):
-        for k in ("KW_NO_QUIET", "KW_NO_GRAPH"):
+        for k in ("KW_NO_LEAN", "KW_NO_GRAPH"):
             monkeypatch.delenv(k, raising=False)
         for k in env:
             monkeypatch.setenv(k, "1")
@@ -127,8 +127,33 @@ def test_quiet_steps_after_sources_end_bit_identical(shape, per_point, monkeypat
             sim.step()
         return sim.run()
 
-    ref = run(("KW_NO_QUIET",))
+    ref = run(("KW_NO_LEAN",))
     assert np.abs(ref["p"]).max() > 0
     for out in (run(()), run((), stepwise=True), run(("KW_NO_GRAPH",))):
         for k in ref:
             assert np.array_equal(ref[k], out[k]), (shape, k)
+
+
+def test_lean_steps_in_a_batch_wait_for_the_longest_source(monkeypatch):
+    """A batch switches to the lean form only after the last of its instances'
+    filtered sources has ended (a dense-pattern source of 29 samples beside a
+    spectral one of 50); every instance keeps the full form's bits."""
+    from kwave_amd.batch import BatchSimulation
+
+    shape = (64, 80)
+    pml, alpha = (4, 4), (2.0, 2.0)
+    cases = [(*_source_case(shape, False, slen=29), pml, alpha), (*_source_case(shape, True, slen=50), pml, alpha)]
+    outs = []
+    for lean in (False, True):
+        monkeypatch.delenv("KW_NO_LEAN", raising=False)
+        if not lean:
+            monkeypatch.setenv("KW_NO_LEAN", "1")
+        bs = BatchSimulation([(kg, med, copy.deepcopy(src), sen, p, a) for kg, med, src, sen, p, a in cases])
+        assert len(bs.plans) == 1
+        try:
+            outs.append(bs.run())
+        finally:
+            bs.close()
+    for full, lean in zip(*outs):
+        for k in full:
+            assert np.array_equal(full[k], lean[k]), k
