@@ -528,3 +528,27 @@ def benchmark(n_train=4096, n_test=512, epochs=80, log=print):
     log(f"surrogate: held-out rel-L2 {err.mean():.4f} (mean-trace baseline {base.mean():.4f}), "
         f"{1e3 * t_sur:.4f} ms/sample vs solver {1e3 * t_sol:.3f} ms/sample -> {t_sol / t_sur:.0f}x")
     return res, model
+
+
+def _main():
+    """`python -m kwave_amd.surrogate --forward [--reps R]`: time the inference forward
+    of an untrained network on 1024 maps (the kernels are the trained network's; used
+    for profiling), or with no flag run the full benchmark."""
+    import argparse
+
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--forward", action="store_true", help="time the forward only (no data, no training)")
+    ap.add_argument("--reps", type=int, default=5)
+    args = ap.parse_args()
+    _torch_first()
+    if not args.forward:
+        benchmark()
+        return
+    model = build_model().eval()
+    cmaps = np.stack([workloads.surrogate_c_map(6_000_000 + i) for i in range(64)]).astype(np.float32)
+    t = time_surrogate(model, cmaps, reps=args.reps)
+    print(f"surrogate forward: {1e3 * t:.4f} ms per map (batch 1024, {args.reps} repetitions)", flush=True)
+
+
+if __name__ == "__main__":
+    _main()
