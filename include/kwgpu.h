@@ -189,6 +189,19 @@ int kw_mode_gemm_bf16(int nmodes, int R, int N, int K, const void* A, const void
 int kw_mode_contract_bf16(int B, int C, int md, int NX, int NY, const void* X, const int64_t xstride[4], const void* Bt, void* Y,
                           const int64_t ystride[4], void* stream);
 
+/* Truncated 2-D DFT of the surrogate's 64 x 64 channel slabs for inference,
+ * both axes in one pass (bf16 operands, FP32 accumulation, the intermediate
+ * rounded to bf16).  S slabs; all arrays device pointers, 16-byte aligned.
+ * dir = 1:  in = v bf16 [S][64][64], m1 = Mx bf16 [64][64] (rows interleave
+ *           re/im of 32 retained kx), m2 = My^T bf16 [32][128];
+ *           out = X fp32 [S][32][32] = (Mx v viewed 32 x 128) My, i.e. complex
+ *           [S][32 kx][16 ky] interleaved (re, im).
+ * dir = -1: in = Y fp32 [S][32][32] (same layout), m1 = Ny^T bf16 [128][32],
+ *           m2 = Nx bf16 [64][64]; out = bf16 [S][64][64] = Nx (Y Ny viewed 64 x 64).
+ * The matrices are those of the network's truncated transforms.  Enqueued on
+ * `stream`.  No counterpart in the reference (it has no surrogate). */
+int kw_fno_dft_bf16(int dir, int64_t S, const void* in, const void* m1, const void* m2, void* out, void* stream);
+
 /* Pointwise part of a surrogate layer for inference, fused into one pass:
  * out[b][o][p] = act(s[b][o][p] + pb[o] + sum_c pw[o][c] v[b][c][p]), with v (the
  * layer input), s (its spectral-conv output) and out bf16 device arrays

@@ -1106,6 +1106,22 @@ int kw_mode_gemm_bf16(int nmodes, int R, int N, int K, const void* A, const void
   return KW_OK;
 }
 
+int kw_fno_dft_bf16(int dir, int64_t S, const void* in, const void* m1, const void* m2, void* out, void* stream) {
+  if (!in || !m1 || !m2 || !out || S < 1) return fail(KW_EINVAL, "FNO DFT: NULL pointer or empty batch");
+  if (dir != 1 && dir != -1) return fail(KW_EINVAL, "FNO DFT: direction must be 1 (forward) or -1 (inverse)");
+  if (S > 0x7fffffff) return fail(KW_EINVAL, "FNO DFT: too many slabs");
+  if (((uintptr_t)in | (uintptr_t)m1 | (uintptr_t)m2 | (uintptr_t)out) & 15) return fail(KW_EINVAL, "FNO DFT: arrays must be 16-byte aligned");
+  hipError_t e;
+  if (dir == 1)
+    e = launch_dft_fwd(S, static_cast<const uint16_t*>(in), static_cast<const uint16_t*>(m1), static_cast<const uint16_t*>(m2),
+                       static_cast<float*>(out), static_cast<hipStream_t>(stream));
+  else
+    e = launch_dft_inv(S, static_cast<const float*>(in), static_cast<const uint16_t*>(m1), static_cast<const uint16_t*>(m2),
+                       static_cast<uint16_t*>(out), static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(KW_EHIP, std::string("FNO DFT launch: ") + hipGetErrorString(e));
+  return KW_OK;
+}
+
 int kw_fno_pointwise_bf16(int B, int C, int64_t P, const void* v, const void* s, const float* pw, const float* pb, void* out, int gelu,
                           void* stream) {
   if (!v || !s || !pw || !pb || !out || B < 1 || P < 4) return fail(KW_EINVAL, "FNO pointwise: NULL pointer or empty shape");

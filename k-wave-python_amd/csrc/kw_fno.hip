@@ -154,6 +154,113 @@ hipError_t launch_fno_pointwise(int B, int C, int64_t P, const uint16_t* v, cons
   return hipGetLastError();
 }
 
+// Truncated 2-D DFT of the surrogate's 64 x 64 channel slabs for inference, both
+// axes in one kernel (one workgroup of four waves per slab, bf16 operands, FP32
+// accumulation on v_mfma_f32_32x32x8_bf16).  With the real DFT matrices of
+// 32 retained kx (rows interleaved re/im) and 16 retained ky:
+//   forward  T = Mx V (64 x 64, rounded to bf16 in LDS), X = T' My with
+//            T' = T viewed 32 x 128 (row j = [T_2j | T_2j+1]); X 32 x 32 fp32,
+//            columns interleaved re/im per ky;
+//   inverse  C = Y Ny (32 x 128, rounded to bf16 in LDS), out = Nx C' with
+//            C' = C viewed 64 x 64; out bf16.
+// My and Ny are passed transposed (MyT 32 x 128, NyT 128 x 32) so every MFMA
+// operand is four consecutive bf16 values.
+namespace {
+constexpr int DN = 64, DK = 32;  // slab edge, retained kx rows (= interleaved ky columns)
+
+// B operand (k = k0 + 4 (l / 32) + 0..3, column n) from a row-major bf16 LDS matrix
+__device__ __forceinline__ bf16x4 lds_col4(const uint16_t* m, int ld, int k, int n) {
+  return bf16x4{(short)m[(k + 0) * ld + n], (short)m[(k + 1) * ld + n], (short)m[(k + 2) * ld + n], (short)m[(k + 3) * ld + n]};
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_dft_fwd(const uint16_t* __restrict__ v, const uint16_t* __restrict__ mx,
+                                                 const uint16_t* __restrict__ myt, float* __restrict__ x) {
+  __shared__ __attribute__((aligned(16))) uint16_t vs[DN * DN];  // the slab
+  __shared__ __attribute__((aligned(16))) uint16_t ts[DN * DN];  // T = Mx V
+  const int64_t slab = blockIdx.x;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int i = l & 31, kh = (l >> 5) * 4;
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(v + slab * DN * DN);
+    uint4* dst = reinterpret_cast<uint4*>(vs);
+    for (int e = tid; e < DN * DN / 8; e += 256) dst[e] = src[e];
+  }
+  __syncthreads();
+  {
+    const int mt = w >> 1, nt = w & 1;
+    f32x16 acc = {};
+#pragma unroll
+    for (int k0 = 0; k0 < DN; k0 += 8) {
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(mx + (mt * 32 + i) * DN + k0 + kh);
+      const bf16x4 b = lds_col4(vs, DN, k0 + kh, nt * 32 + i);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) ts[(mt * 32 + 8 * (r >> 2) + kh + (r & 3)) * DN + nt * 32 + i] = (uint16_t)bf16_rne(acc[r]);
+  }
+  __syncthreads();
+  if (w == 0) {
+    // T' row j = T rows 2j, 2j+1: element (j, q) sits at ts[j * 2 DN + q]
+    f32x16 acc = {};
+#pragma unroll
+    for (int k0 = 0; k0 < 2 * DN; k0 += 8) {
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(ts + i * 2 * DN + k0 + kh);
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(myt + i * 2 * DN + k0 + kh);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    float* xo = x + slab * DK * DK;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) xo[(8 * (r >> 2) + kh + (r & 3)) * DK + i] = acc[r];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dft_inv(const float* __restrict__ y, const uint16_t* __restrict__ nyt,
+                                                 const uint16_t* __restrict__ nx, uint16_t* __restrict__ out) {
+  __shared__ __attribute__((aligned(16))) uint16_t cs[DK * 2 * DN];  // C = Y Ny, 32 x 128 (= C' 64 x 64)
+  const int64_t slab = blockIdx.x;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int i = l & 31, kh = (l >> 5) * 4;
+  {
+    // wave w: columns [32 w, 32 w + 32) of C, K = 32
+    const float* ys = y + slab * DK * DK;
+    f32x16 acc = {};
+#pragma unroll
+    for (int k0 = 0; k0 < DK; k0 += 8) {
+      const float4 yv = *reinterpret_cast<const float4*>(ys + i * DK + k0 + kh);
+      const bf16x4 a = bf16x4{bf16_rne(yv.x), bf16_rne(yv.y), bf16_rne(yv.z), bf16_rne(yv.w)};
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(nyt + (w * 32 + i) * DK + k0 + kh);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) cs[(8 * (r >> 2) + kh + (r & 3)) * 2 * DN + w * 32 + i] = (uint16_t)bf16_rne(acc[r]);
+  }
+  __syncthreads();
+  {
+    const int mt = w >> 1, nt = w & 1;
+    f32x16 acc = {};
+#pragma unroll
+    for (int k0 = 0; k0 < DN; k0 += 8) {
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(nx + (mt * 32 + i) * DN + k0 + kh);
+      const bf16x4 b = lds_col4(cs, DN, k0 + kh, nt * 32 + i);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    uint16_t* o = out + slab * DN * DN;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) o[(mt * 32 + 8 * (r >> 2) + kh + (r & 3)) * DN + nt * 32 + i] = (uint16_t)bf16_rne(acc[r]);
+  }
+}
+
+hipError_t launch_dft_fwd(int64_t S, const uint16_t* v, const uint16_t* mx, const uint16_t* myt, float* x, hipStream_t st) {
+  hipLaunchKernelGGL(k_dft_fwd, dim3((unsigned)S), dim3(256), 0, st, v, mx, myt, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_dft_inv(int64_t S, const float* y, const uint16_t* nyt, const uint16_t* nx, uint16_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_dft_inv, dim3((unsigned)S), dim3(256), 0, st, y, nyt, nx, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_mode_gemm(int M, int R, int N, int K, const uint16_t* A, const uint16_t* Bt, float* C, hipStream_t st) {
   hipLaunchKernelGGL(k_mode_gemm, dim3(N / 32, R / 32, M), dim3(64), 0, st, R, N, K, A, Bt, C);
   return hipGetLastError();

@@ -19,8 +19,9 @@ gradient - runs in the library's BF16 MFMA kernels (kw_mode_contract_bf16,
 kw_mode_gemm_bf16, FP32 accumulation).  Only 2 md x md of the 64 x 33 Fourier
 modes are kept, so the truncated transforms are dense matrix products (two per
 direction, one per axis) rather than full FFTs; pointwise layers and the
-optimiser are PyTorch; at inference each layer's pointwise part (channel mix,
-bias, residual sum, GELU) is one fused kernel (kw_fno_pointwise_bf16).
+optimiser are PyTorch; at inference both truncated transforms of a layer run in
+one MFMA kernel (kw_fno_dft_bf16) and its pointwise part (channel mix, bias,
+residual sum, GELU) in another (kw_fno_pointwise_bf16).
 """
 from __future__ import annotations
 
@@ -128,6 +129,30 @@ def fno_pointwise(v, s, pw, pb, gelu):
     _abi.check(_abi.lib().kw_fno_pointwise_bf16(b, c, p, C.c_void_p(v.data_ptr()), C.c_void_p(s.data_ptr()),
                                                 C.c_void_p(pw.data_ptr()), C.c_void_p(pb.data_ptr()),
                                                 C.c_void_p(out.data_ptr()), int(bool(gelu)), C.c_void_p(stream)))
+    return out
+
+
+def fno_dft(direction, x, m1, m2):
+    """Both truncated transforms of 64 x 64 slabs in one kernel (inference; kw_fno_dft_bf16).
+
+    direction 1: x bf16 (..., 64, 64), m1 = Mx, m2 = My^T (bf16) -> float32 (..., 32, 32),
+    the retained modes with (re, im) interleaved along the last axis.
+    direction -1: x float32 (..., 32, 32) in that layout, m1 = Ny^T, m2 = Nx -> bf16 (..., 64, 64).
+    """
+    import torch
+
+    fwd = direction == 1
+    shape_in, shape_out = ((64, 64), (32, 32)) if fwd else ((32, 32), (64, 64))
+    dt_in, dt_out = (torch.bfloat16, torch.float32) if fwd else (torch.float32, torch.bfloat16)
+    if not x.is_cuda or x.dtype != dt_in or tuple(x.shape[-2:]) != shape_in:
+        raise ValueError(f"fno_dft({direction}) takes {dt_in} device slabs of {shape_in}")
+    x = x.contiguous()
+    lead = x.shape[:-2]
+    out = torch.empty(*lead, *shape_out, dtype=dt_out, device=x.device)
+    stream = torch.cuda.current_stream(x.device).cuda_stream
+    _abi.check(_abi.lib().kw_fno_dft_bf16(int(direction), x.numel() // (shape_in[0] * shape_in[1]), C.c_void_p(x.data_ptr()),
+                                          C.c_void_p(m1.data_ptr()), C.c_void_p(m2.data_ptr()), C.c_void_p(out.data_ptr()),
+                                          C.c_void_p(stream)))
     return out
 
 
@@ -256,8 +281,14 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
             scale = 1.0 / (c * c)
             self.wr = nn.Parameter(scale * torch.randn(2 * modes * modes, c, c))
             self.wi = nn.Parameter(scale * torch.randn(2 * modes * modes, c, c))
-            for name, a in zip(("mx", "my", "ny", "nx"), dft_mats(n, modes)):
+            mats = dft_mats(n, modes)
+            for name, a in zip(("mx", "my", "ny", "nx"), mats):
                 self.register_buffer(name, torch.as_tensor(a))
+            # bf16 operands of the fused inference transforms (kw_fno_dft_bf16: 64 x 64 slabs, 16 modes)
+            self.fused_dft = n == 64 and modes == 16
+            mx, my, ny_, nx_ = mats
+            for name, a in (("mx_b", mx), ("myt_b", my.T), ("nyt_b", ny_.T), ("nx_b", nx_)):
+                self.register_buffer(name, torch.as_tensor(np.ascontiguousarray(a)).to(torch.bfloat16), persistent=False)
 
         def weights(self):
             c = self.c
@@ -267,6 +298,10 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
         def forward(self, v):
             b, c, n, _ = v.shape
             md = self.modes
+            if self.fused_dft and v.dtype == torch.bfloat16 and v.is_cuda and not torch.is_grad_enabled():
+                xm = fno_dft(1, v, self.mx_b, self.myt_b).reshape(b, c, 2 * md, md, 2)
+                ym = spectral_modes(torch.view_as_complex(xm), self.weights(), md)
+                return fno_dft(-1, torch.view_as_real(ym).reshape(b, c, 2 * md, 2 * md), self.nyt_b, self.nx_b)
             bx = torch.matmul(self.mx, v).reshape(b, c, 2 * md, 2 * n)
             xm = torch.matmul(bx, self.my).float().reshape(b, c, 2 * md, md, 2)  # retained modes
             ym = spectral_modes(torch.view_as_complex(xm), self.weights(), md)
