@@ -167,24 +167,31 @@ hipError_t launch_fno_pointwise(int B, int C, int64_t P, const uint16_t* v, cons
 // operand is four consecutive bf16 values.
 namespace {
 constexpr int DN = 64, DK = 32;  // slab edge, retained kx rows (= interleaved ky columns)
-
-// B operand (k = k0 + 4 (l / 32) + 0..3, column n) from a row-major bf16 LDS matrix
-__device__ __forceinline__ bf16x4 lds_col4(const uint16_t* m, int ld, int k, int n) {
-  return bf16x4{(short)m[(k + 0) * ld + n], (short)m[(k + 1) * ld + n], (short)m[(k + 2) * ld + n], (short)m[(k + 3) * ld + n]};
-}
+// LDS copies of B operands are stored transposed (column n contiguous in k) with
+// this row pitch: four k values are one 8-byte read, and the pad spreads the
+// transposing writes over the banks.
+constexpr int LDT = DN + 4;
 }  // namespace
 
 __global__ __launch_bounds__(256) void k_dft_fwd(const uint16_t* __restrict__ v, const uint16_t* __restrict__ mx,
                                                  const uint16_t* __restrict__ myt, float* __restrict__ x) {
-  __shared__ __attribute__((aligned(16))) uint16_t vs[DN * DN];  // the slab
-  __shared__ __attribute__((aligned(16))) uint16_t ts[DN * DN];  // T = Mx V
+  __shared__ __attribute__((aligned(16))) uint16_t vt[DN * LDT];  // the slab, transposed: vt[y][x]
+  __shared__ __attribute__((aligned(16))) uint16_t ts[DN * DN];   // T = Mx V
   const int64_t slab = blockIdx.x;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int i = l & 31, kh = (l >> 5) * 4;
   {
     const uint4* src = reinterpret_cast<const uint4*>(v + slab * DN * DN);
-    uint4* dst = reinterpret_cast<uint4*>(vs);
-    for (int e = tid; e < DN * DN / 8; e += 256) dst[e] = src[e];
+    for (int e = tid; e < DN * DN / 8; e += 256) {
+      const uint4 q = src[e];
+      const int xr = e >> 3, y0 = (e & 7) * 8;
+      const unsigned u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        vt[(y0 + 2 * j) * LDT + xr] = (uint16_t)(u[j] & 0xffffu);
+        vt[(y0 + 2 * j + 1) * LDT + xr] = (uint16_t)(u[j] >> 16);
+      }
+    }
   }
   __syncthreads();
   {
@@ -193,7 +200,7 @@ __global__ __launch_bounds__(256) void k_dft_fwd(const uint16_t* __restrict__ v,
 #pragma unroll
     for (int k0 = 0; k0 < DN; k0 += 8) {
       const bf16x4 a = *reinterpret_cast<const bf16x4*>(mx + (mt * 32 + i) * DN + k0 + kh);
-      const bf16x4 b = lds_col4(vs, DN, k0 + kh, nt * 32 + i);
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(vt + (nt * 32 + i) * LDT + k0 + kh);
       acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
     }
 #pragma unroll
@@ -217,7 +224,7 @@ __global__ __launch_bounds__(256) void k_dft_fwd(const uint16_t* __restrict__ v,
 
 __global__ __launch_bounds__(256) void k_dft_inv(const float* __restrict__ y, const uint16_t* __restrict__ nyt,
                                                  const uint16_t* __restrict__ nx, uint16_t* __restrict__ out) {
-  __shared__ __attribute__((aligned(16))) uint16_t cs[DK * 2 * DN];  // C = Y Ny, 32 x 128 (= C' 64 x 64)
+  __shared__ __attribute__((aligned(16))) uint16_t ct[DN * LDT];  // C' = (Y Ny viewed 64 x 64), transposed: ct[n][k]
   const int64_t slab = blockIdx.x;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int i = l & 31, kh = (l >> 5) * 4;
@@ -232,8 +239,10 @@ __global__ __launch_bounds__(256) void k_dft_inv(const float* __restrict__ y, co
       const bf16x4 b = *reinterpret_cast<const bf16x4*>(nyt + (w * 32 + i) * DK + k0 + kh);
       acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
     }
+    // C[j][q] is C'[2 j + q / 64][q % 64], with q = 32 w + i
+    const int n = (w & 1) * 32 + i, kk = w >> 1;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) cs[(8 * (r >> 2) + kh + (r & 3)) * 2 * DN + w * 32 + i] = (uint16_t)bf16_rne(acc[r]);
+    for (int r = 0; r < 16; ++r) ct[n * LDT + 2 * (8 * (r >> 2) + kh + (r & 3)) + kk] = (uint16_t)bf16_rne(acc[r]);
   }
   __syncthreads();
   {
@@ -242,7 +251,7 @@ __global__ __launch_bounds__(256) void k_dft_inv(const float* __restrict__ y, co
 #pragma unroll
     for (int k0 = 0; k0 < DN; k0 += 8) {
       const bf16x4 a = *reinterpret_cast<const bf16x4*>(nx + (mt * 32 + i) * DN + k0 + kh);
-      const bf16x4 b = lds_col4(cs, DN, k0 + kh, nt * 32 + i);
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(ct + (nt * 32 + i) * LDT + k0 + kh);
       acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
     }
     uint16_t* o = out + slab * DN * DN;
