@@ -202,6 +202,13 @@ int kw_mode_contract_bf16(int B, int C, int md, int NX, int NY, const void* X, c
  * `stream`.  No counterpart in the reference (it has no surrogate). */
 int kw_fno_dft_bf16(int dir, int64_t S, const void* in, const void* m1, const void* m2, void* out, void* stream);
 
+/* Sensor-head gather of the surrogate for inference: sel[b][k][c][x] =
+ * v[b][c][x][cols[k]] for b < B, k < K, c < C, x < 64, with v bf16 [B][C][64][64],
+ * cols int32 [K] (device, every value in [0, 64): the caller checks), sel bf16
+ * [B][K][C][64]; device pointers.  Enqueued on `stream`.  No counterpart in the
+ * reference (it has no surrogate). */
+int kw_fno_head_gather_bf16(int64_t B, int C, int K, const void* v, const int* cols, void* sel, void* stream);
+
 /* Pointwise part of a surrogate layer for inference, fused into one pass:
  * out[b][o][p] = act(s[b][o][p] + pb[o] + sum_c pw[o][c] v[b][c][p]), with v (the
  * layer input), s (its spectral-conv output) and out bf16 device arrays

@@ -1106,6 +1106,15 @@ int kw_mode_gemm_bf16(int nmodes, int R, int N, int K, const void* A, const void
   return KW_OK;
 }
 
+int kw_fno_head_gather_bf16(int64_t B, int C, int K, const void* v, const int* cols, void* sel, void* stream) {
+  if (!v || !cols || !sel || B < 1 || C < 1 || K < 1) return fail(KW_EINVAL, "FNO head gather: NULL pointer or empty shape");
+  if (B * C > 0x7fffffff) return fail(KW_EINVAL, "FNO head gather: too many slabs");
+  hipError_t e = launch_head_gather(B, C, K, static_cast<const uint16_t*>(v), cols, static_cast<uint16_t*>(sel),
+                                    static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(KW_EHIP, std::string("FNO head gather launch: ") + hipGetErrorString(e));
+  return KW_OK;
+}
+
 int kw_fno_dft_bf16(int dir, int64_t S, const void* in, const void* m1, const void* m2, void* out, void* stream) {
   if (!in || !m1 || !m2 || !out || S < 1) return fail(KW_EINVAL, "FNO DFT: NULL pointer or empty batch");
   if (dir != 1 && dir != -1) return fail(KW_EINVAL, "FNO DFT: direction must be 1 (forward) or -1 (inverse)");

@@ -260,6 +260,32 @@ __global__ __launch_bounds__(256) void k_dft_inv(const float* __restrict__ y, co
   }
 }
 
+// Sensor-head gather of the surrogate: sel[b][k][c][x] = v[b][c][x][cols[k]] for
+// k < K (bf16, v [B][C][64][64]).  One workgroup per (b, c) slab: the slab is
+// staged in LDS with a padded row pitch (the column reads of different rows
+// fall in different banks), then written out one contiguous row of 64 x per k.
+__global__ __launch_bounds__(256) void k_head_gather(int C, int K, const uint16_t* __restrict__ v, const int* __restrict__ cols,
+                                                     uint16_t* __restrict__ sel) {
+  constexpr int LDP = DN + 2;
+  __shared__ uint16_t vs[DN * LDP];
+  const int64_t slab = blockIdx.x;  // b * C + c
+  const int64_t b = slab / C, c = slab - b * C;
+  const int tid = threadIdx.x;
+  const uint16_t* src = v + slab * DN * DN;
+  for (int e = tid; e < DN * DN; e += 256) vs[(e >> 6) * LDP + (e & 63)] = src[e];
+  __syncthreads();
+  for (int e = tid; e < K * DN; e += 256) {
+    const int k = e >> 6, xr = e & 63;
+    const int y = min(max(cols[k], 0), DN - 1);  // the caller checks the range; this keeps reads in the slab
+    sel[((b * K + k) * C + c) * DN + xr] = vs[xr * LDP + y];
+  }
+}
+
+hipError_t launch_head_gather(int64_t B, int C, int K, const uint16_t* v, const int* cols, uint16_t* sel, hipStream_t st) {
+  hipLaunchKernelGGL(k_head_gather, dim3((unsigned)(B * C)), dim3(256), 0, st, C, K, v, cols, sel);
+  return hipGetLastError();
+}
+
 hipError_t launch_dft_fwd(int64_t S, const uint16_t* v, const uint16_t* mx, const uint16_t* myt, float* x, hipStream_t st) {
   hipLaunchKernelGGL(k_dft_fwd, dim3((unsigned)S), dim3(256), 0, st, v, mx, myt, x);
   return hipGetLastError();

@@ -156,6 +156,24 @@ def fno_dft(direction, x, m1, m2):
     return out
 
 
+def head_gather(v, cols):
+    """sel[b, k] = v[b, :, :, cols[k]] as (B, K, C, 64) bf16 (inference; kw_fno_head_gather_bf16).
+
+    v: bf16 (B, C, 64, 64) device tensor; cols: int32 device tensor of column indices in [0, 64).
+    """
+    import torch
+
+    if v.dtype != torch.bfloat16 or not v.is_cuda or tuple(v.shape[-2:]) != (64, 64) or cols.dtype != torch.int32:
+        raise ValueError("head_gather takes bf16 (B, C, 64, 64) slabs and int32 column indices")
+    v, cols = v.contiguous(), cols.contiguous()
+    b, c = v.shape[:2]
+    sel = torch.empty(b, cols.numel(), c, 64, dtype=torch.bfloat16, device=v.device)
+    stream = torch.cuda.current_stream(v.device).cuda_stream
+    _abi.check(_abi.lib().kw_fno_head_gather_bf16(b, c, cols.numel(), C.c_void_p(v.data_ptr()), C.c_void_p(cols.data_ptr()),
+                                                  C.c_void_p(sel.data_ptr()), C.c_void_p(stream)))
+    return sel
+
+
 def gather_modes(x, md):
     """Retained modes of a spectrum as interleaved real pairs, [2 md md, B, 2C]."""
     import torch
@@ -325,7 +343,11 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
             nn.init.zeros_(self.head_b.weight)
             nn.init.zeros_(self.head_f.weight)
             self.mean = nn.Parameter(torch.zeros(n_sens, nt))
-            self.register_buffer("cols_ab", torch.as_tensor(np.stack([cols_a, cols_b], 1).reshape(-1), dtype=torch.long))
+            ab = np.stack([cols_a, cols_b], 1).reshape(-1)
+            if ab.min() < 0 or ab.max() >= N_COARSE:
+                raise ValueError("sensor columns outside the coarse grid")
+            self.register_buffer("cols_ab", torch.as_tensor(ab, dtype=torch.long))
+            self.register_buffer("cols_ab32", torch.as_tensor(ab, dtype=torch.int32), persistent=False)
             self.register_buffer("cols_f", torch.as_tensor(cols_f, dtype=torch.long))
             g = torch.linspace(0, 1, N_COARSE)
             self.register_buffer("grid", torch.stack(torch.meshgrid(g, g, indexing="ij"))[None])
@@ -351,7 +373,10 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
                 if i < self.layers - 1:
                     v = F.gelu(v)
             # per sensor s the feature rows [column a_s | column b_s] (2 x wd x n), one product for both heads
-            sel = v.permute(0, 3, 1, 2).index_select(1, self.cols_ab).reshape(b * self.n_sens, 2 * wd * n)
+            if fused:
+                sel = head_gather(v, self.cols_ab32).reshape(b * self.n_sens, 2 * wd * n)
+            else:
+                sel = v.permute(0, 3, 1, 2).index_select(1, self.cols_ab).reshape(b * self.n_sens, 2 * wd * n)
             wab = torch.cat([self.head_a.weight, self.head_b.weight], 1)
             ha = F.linear(sel, wab, self.head_a.bias).reshape(b, self.n_sens, -1)
             ff = cn.index_select(2, self.cols_f).permute(0, 2, 1)                # (b, n_sens, 128)

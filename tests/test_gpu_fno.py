@@ -215,3 +215,17 @@ def test_fused_dft_inverse_matches_irfft2_and_bf16_emulation():
     c = (y.to(torch.bfloat16).float() @ nytb.float().T).to(torch.bfloat16).float().reshape(3, 16, 64, 64)
     emu = (nxb.float() @ c).cpu().double()
     assert _rel(out.numpy(), emu.numpy()) < 2e-3
+
+
+def test_head_gather_matches_index_select():
+    """kw_fno_head_gather_bf16 moves bf16 values: bit-identical to PyTorch's gather."""
+    import torch
+
+    from kwave_amd.surrogate import head_gather
+
+    g = torch.Generator().manual_seed(5)
+    v = torch.randn(3, 16, 64, 64, generator=g).to("cuda", torch.bfloat16)
+    cols = torch.tensor([0, 63, 5, 5, 17, 40, 62, 1], dtype=torch.int32)
+    got = head_gather(v, cols.cuda())
+    want = v.permute(0, 3, 1, 2).index_select(1, cols.long().cuda())
+    assert torch.equal(got, want)
