@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # fused-minimum algorithmic bytes per voxel-step of this workload: 412 B
 # (3-D transforms + fused pointwise, power-law, nonlinear) plus 60 B on the
 # source's ~440 active steps of 1500
-SURVEY_B_ALG = 412.0 + 60.0 * 440.0 / 1500.0
+STEP_MODEL_BYTES = 412.0 + 60.0 * 440.0 / 1500.0
 
 
 PROFILE = ROOT / "profiles" / "rocprof_3d_nonlinear_512.md"
@@ -274,7 +274,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, n),
                      "alg_bytes_per_launch": by / nl},
-        "hbm_frac_step_model": SURVEY_B_ALG * rate / (HBM_PEAK_GBS * 1e9),
+        "hbm_frac_step_model": STEP_MODEL_BYTES * rate / (HBM_PEAK_GBS * 1e9),
         "alg_bytes_per_voxel_step": alg_bytes_step / ntot,
         "gpu_ms_per_step": gpu_ms_step,
         "kernels": kernels,

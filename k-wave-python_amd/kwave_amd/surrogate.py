@@ -2,8 +2,8 @@
 
 The problem is ``workloads.surrogate_2d``: a 128^2 c map in [1400, 1650] m/s, a
 32-cell additive line source with a coincident 32-cell receive line, 600 steps.
-Training data come from the GPU solver (batched plans); the solver is also the
-judge of the surrogate's error and its fallback.
+Training data come from the GPU solver (batched plans); the solver also measures
+the surrogate's error and is its fallback.
 
 Network (runs at 64^2, the c map averaged over 2 x 2 cells):
   lift     (c, x, y) -> 16 channels, pointwise
