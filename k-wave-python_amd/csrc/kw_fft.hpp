@@ -175,9 +175,10 @@ __device__ __forceinline__ void dft(float2* v) {
 // LDS index maps used for the exchanges
 // ---------------------------------------------------------------------------
 // lines-fastest (column passes: consecutive threads = consecutive columns)
-// Row pitch L + pad: at 512^3 a pad of 2 for the 8- and 4-wide tiles cut the
-// x-pass times by up to 5 % against a pad of 1 (3 and 5 were in between); the
-// 16-wide tiles keep 1 (their bank conflicts are already near zero).
+// Row pitch L + pad.  The 8-wide x-pass tiles had 2.8 LDS bank-conflict cycles
+// per LDS instruction at pad 1; pad 2 measured best of 1, 2, 3 and 5 at 512^3
+// (0-5 % on the x-passes, about the box-to-box variation).  The 16-wide tiles
+// keep 1 (their conflicts are already near zero).
 #ifndef KW_COLPAD8
 #define KW_COLPAD8 2
 #endif
