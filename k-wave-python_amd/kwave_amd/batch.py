@@ -13,6 +13,7 @@ from __future__ import annotations
 
 from collections import OrderedDict
 
+from . import _abi
 from .solvers.kspace_solver import _resolve_dtype, collect_results
 from .solvers.plan import DevicePlan
 from .solvers.setup import build_instance
@@ -27,7 +28,9 @@ class BatchSimulation:
                                       pml_size=pml, pml_alpha=alpha) for kg, med, src, sen, pml, alpha in cases]
         groups = OrderedDict()
         for i, st in enumerate(self.setups):
-            key = (st.shape, tuple(st.spacing), st.flags, st.use_sg, st.p0 is not None)
+            # the power-law exponent is a per-plan constant of the fractional Laplacians
+            y = st.alpha_power if st.flags & (_abi.KW_STOKES | _abi.KW_POWERLAW_ABS | _abi.KW_POWERLAW_DISP) else None
+            key = (st.shape, tuple(st.spacing), st.flags, st.use_sg, st.p0 is not None, y)
             groups.setdefault(key, []).append(i)
         self.groups = list(groups.values())
         self.plans = [DevicePlan([self.setups[i] for i in idx], device=device) for idx in self.groups]

@@ -9,6 +9,9 @@ from .. import _abi
 from .._abi import check, f32, fptr
 
 
+_ABSORBING = _abi.KW_STOKES | _abi.KW_POWERLAW_ABS | _abi.KW_POWERLAW_DISP
+
+
 def _i64(a):
     a = np.ascontiguousarray(a, dtype=np.int64)
     return a, a.ctypes.data_as(C.POINTER(C.c_int64))
@@ -26,6 +29,10 @@ class DevicePlan:
                 raise ValueError("all instances of a plan need the same physics switches")
             if (s.p0 is None) != (s0.p0 is None):
                 raise ValueError("either every instance of a plan has p0 or none has")
+            if (s0.flags & _ABSORBING) and s.alpha_power != s0.alpha_power:
+                # the fractional-Laplacian exponents y-2, y-1 and tan(pi y / 2) are set once per plan
+                raise ValueError("all absorbing instances of a plan need the same alpha_power "
+                                 f"(got {s0.alpha_power} and {s.alpha_power})")
         L = _abi.lib()
         d = _abi.PlanDesc()
         d.ndim = s0.ndim

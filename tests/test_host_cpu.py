@@ -144,3 +144,19 @@ def test_unified_api_argument_errors():
     assert _resolve_dtype("single") is np.float32 and _resolve_dtype(None) is np.float64
     with pytest.raises(ValueError):
         _resolve_dtype(np.int32)
+
+
+def test_plan_refuses_mixed_alpha_power():
+    """The power-law exponent is a per-plan constant: a plan of absorbing instances
+    with different alpha_power is refused before any device call."""
+    from kwave_amd import workloads
+    from kwave_amd.solvers.plan import DevicePlan
+    from kwave_amd.solvers.setup import build_instance
+
+    sts = []
+    for y in (1.1, 1.5):
+        kg, med, src, sen, pml, alpha = workloads.hetero_2d(seed=40, n=64, nt=20, pml=8)
+        med.alpha_power = y
+        sts.append(build_instance(kg, med, src, sen, pml_size=pml, pml_alpha=alpha))
+    with pytest.raises(ValueError, match="alpha_power"):
+        DevicePlan(sts)
