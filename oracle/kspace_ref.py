@@ -15,6 +15,14 @@ that its cost also models the reference's CPU path.
 Pins (see tests/test_oracle_*.py):
   * closed form: homogeneous lossless IVP with PML off equals
     ``F^-1[cos(n c|k| dt) F[p0]]`` (exactness of the k-space leapfrog);
+  * physics of plane waves (tests/test_oracle_physics.py): power-law decay
+    ``exp(-alpha_0 w^y x)`` for y = 1.1, 1.5 and Stokes y = 2, the
+    Kramers-Kronig phase speed of kwave/utils/mapgen.py:328-358, second-
+    harmonic growth of the Fubini solution with beta = 1 + B/2A (the B/A term
+    and the (2 sum(rho) + rho0)/rho0 mass factor), the intensity
+    ``I_avg = P^2 / (2 rho c)`` and the near-unity ``alpha_power`` stability
+    case of tests/test_issue_664_alpha_power_near_unity.py, each with a planted
+    mistake that the check must catch;
   * ``get_pml`` closed form and reference test values (tests/test_pmlutils.py);
   * ``db2neper`` known answer (tests/test_utils.py:31-35);
   * ``smooth``/``make_disc``/``_stagger`` through the reference's SHA-256
@@ -439,6 +447,11 @@ class OracleSimulation:
         """kspace_solver.py:781-796."""
         return np.real(np.fft.ifftn(op * np.fft.fftn(f)))
 
+    def _mass_factor(self):
+        """(2 sum(rho_i) + rho0) / rho0 of the nonlinear mass equation, 1 when linear;
+        kspace_solver.py:549-550, 717."""
+        return (2 * sum(self.rho) + self.rho0) / self.rho0 if self.BonA is not None else 1.0
+
     def step(self):
         if self.t >= self.Nt:
             return self
@@ -448,7 +461,7 @@ class OracleSimulation:
             g = np.real(np.fft.ifftn(self.op_grad[i] * P))
             self.u[i] = self.pml_sg[i] * (self.pml_sg[i] * self.u[i] - self.dt_rho0[i] * g)
             self.u[i] = self.src_u[i](self.t, self.u[i])
-        nl = (2 * sum(self.rho) + self.rho0) / self.rho0 if self.BonA is not None else 1.0
+        nl = self._mass_factor()
         div_total = np.zeros(self.shape)
         for i in range(nd):
             d = self._diff(self.u[i], self.op_div[i])
@@ -512,12 +525,16 @@ def compute_aggregates(result, ndim, record):
     return out
 
 
+def _half_step_shift(n):
+    """Spectral multiplier advancing a length-n series by half a sample; kspace_solver.py:846-849."""
+    freq = 2 * np.pi * np.arange(-(n // 2), n - n // 2) / n
+    return np.fft.ifftshift(np.exp(1j * freq * 0.5))
+
+
 def acoustic_intensity(result):
     """Temporal half-step Fourier shift of u, then I = p u; kspace_solver.py:840-862."""
     p = result["p"]
-    n = p.shape[-1]
-    freq = 2 * np.pi * np.arange(-(n // 2), n - n // 2) / n
-    shift = np.fft.ifftshift(np.exp(1j * freq * 0.5))
+    shift = _half_step_shift(p.shape[-1])
     out = {}
     for a in "xyz":
         u = result.get(f"u{a}")
