@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/kwgpu.h"
+#include "kw_host.hpp"
 #include "kw_launch.hpp"
 #include "kw_types.hpp"
 
@@ -61,66 +62,11 @@ struct DevBuf {
   }
 };
 
-// 1-D operator tables for one axis (FP64 construction, FP32 storage)
-struct AxisTables {
-  std::vector<float2> grad, div, unst, tw;
-  std::vector<float> k;
-};
-
-AxisTables build_axis(int n, double d, bool use_sg) {
-  AxisTables t;
-  t.grad.resize(n); t.div.resize(n); t.unst.resize(n); t.tw.resize(n); t.k.resize(n);
-  const double two_pi = 2.0 * M_PI;
-  for (int i = 0; i < n; ++i) {
-    // numpy.fft.fftfreq ordering: 0..ceil(n/2)-1, then -floor(n/2)..-1
-    const int f = (i < (n + 1) / 2) ? i : i - n;
-    const double k = two_pi * f / (n * d);
-    std::complex<double> ik(0.0, k);
-    std::complex<double> sp = use_sg ? std::exp(std::complex<double>(0.0, k * d / 2)) : 1.0;
-    std::complex<double> sm = use_sg ? std::exp(std::complex<double>(0.0, -k * d / 2)) : 1.0;
-    std::complex<double> g = ik * sp, dv = ik * sm, un = std::exp(std::complex<double>(0.0, -k * d / 2));
-    if (n % 2 == 0 && i == n / 2) {  // self-conjugate bin: keep the Hermitian part
-      g = g.real(); dv = dv.real(); un = un.real();
-    }
-    t.grad[i] = make_float2((float)g.real(), (float)g.imag());
-    t.div[i] = make_float2((float)dv.real(), (float)dv.imag());
-    t.unst[i] = make_float2((float)un.real(), (float)un.imag());
-    t.k[i] = (float)k;
-    const double a = -two_pi * i / n;
-    t.tw[i] = make_float2((float)std::cos(a), (float)std::sin(a));
-  }
-  return t;
-}
-
-Geom make_geom(int nd, const int* n) {
-  Geom g{};
-  g.nd = nd;
-  g.n[0] = n[0]; g.n[1] = n[1]; g.n[2] = nd == 3 ? n[2] : 1;
-  g.nz = n[nd - 1];
-  g.nh = g.nz / 2 + 1;
-  g.nhp = (g.nh + 15) & ~15;
-  g.ntot = (int64_t)g.n[0] * g.n[1] * g.n[2];
-  g.rows = (int)(g.ntot / g.nz);
-  g.nspec = (int64_t)g.rows * g.nhp;
-  return g;
-}
-
-ColArgs col_args(const Geom& g, int axis, int mode) {
-  ColArgs a{};
-  a.mode = mode;
-  a.axis = axis;
-  a.ncols = g.nh;
-  if (g.nd == 2) {
-    a.estride = g.nhp; a.ostride = 0; a.nouter = 1;
-  } else if (axis == 1) {
-    a.estride = g.nhp; a.ostride = (int64_t)g.n[1] * g.nhp; a.nouter = g.n[0];
-  } else {
-    a.estride = (int64_t)g.n[1] * g.nhp; a.ostride = g.nhp; a.nouter = g.n[1];
-  }
-  return a;
-}
-
 }  // namespace
+
+namespace kw {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace kw
 
 struct kw_plan {
   int dev = 0;
@@ -156,6 +102,7 @@ struct kw_plan {
   bool skip_gather = false;  // replayed steps of the reverse sweep do not record
   int col_tile[2] = {0, 0};  // tile variant of the FFT-only and the spectral-job column passes
   bool tuned = false;
+  bool col_wide = false;  // KW_COL_WIDE=1: no narrow-tile fallback on small grids
   bool inst_dirty = true;
   // per-kernel timing (kw_plan_run_profiled)
   bool prof = false;
@@ -256,6 +203,7 @@ int ypass(kw_plan* P, int mode, const std::vector<int>& bufs) {
   if (P->g.nd != 3 || bufs.empty()) return KW_OK;
   ColArgs a = col_args(P->g, 1, mode);
   a.tile = P->col_tile[0];
+  a.wide = P->col_wide;
   a.nbuf = (int)bufs.size();
   for (size_t i = 0; i < bufs.size(); ++i) a.bufs[i] = bufs[i];
   const double spec_bytes = 8.0 * P->nb * (double)P->g.rows * P->g.nh;
@@ -266,6 +214,7 @@ int ypass(kw_plan* P, int mode, const std::vector<int>& bufs) {
 int xpass(kw_plan* P, const std::vector<XJob>& jobs, int kind) {
   ColArgs a = col_args(P->g, 0, 2);
   a.tile = P->col_tile[1];
+  a.wide = P->col_wide;
   a.njob = (int)jobs.size();
   int nin = 0, last = -1;
   for (size_t i = 0; i < jobs.size(); ++i) {
@@ -463,6 +412,10 @@ int kw_plan_create(const kw_plan_desc* d, kw_plan** out) {
   for (int i = 0; i < 3; ++i) P->spacing[i] = d->spacing[i];
   P->graphs = std::getenv("KW_NO_GRAPH") == nullptr;
   P->lean = std::getenv("KW_NO_LEAN") == nullptr;
+  {
+    const char* w = std::getenv("KW_COL_WIDE");
+    P->col_wide = w && std::atoi(w) != 0;
+  }
   int rc = KW_OK;
   auto cleanup = [&](int code) { delete P; return code; };
   if (hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking) != hipSuccess) return cleanup(fail(KW_EHIP, "stream creation failed"));

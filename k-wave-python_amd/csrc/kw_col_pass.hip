@@ -5,13 +5,13 @@
 namespace kw {
 
 // One launch at tile width LW: wide tiles when they still give at least 1024
-// workgroups, the narrow col_lsmall width otherwise (small grids).
+// workgroups (or a.wide is set), the narrow col_lsmall width otherwise (small grids).
 template <int S, int LW, bool OPS>
 static hipError_t col_launch(const Geom& g, int nb, const Fields& fl, const Ops& ops, const Inst* inst, const ColArgs& a,
                              hipStream_t st) {
   constexpr int LS = col_lsmall<S>();
   const int ntw = (a.ncols + LW - 1) / LW;
-  if ((int64_t)a.nouter * nb * ntw >= 1024 || LW == LS) {
+  if ((int64_t)a.nouter * nb * ntw >= 1024 || LW == LS || a.wide) {
     hipLaunchKernelGGL((k_col<S, LW, OPS>), dim3(a.nouter * ntw, nb), dim3(ColCfg<S, LW>::NT), 0, st, g, fl, ops, inst, a);
   } else {
     const int nt = (a.ncols + LS - 1) / LS;

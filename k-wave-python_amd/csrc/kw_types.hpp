@@ -101,6 +101,7 @@ struct ColArgs {
   int nouter;       // outer index count (3-D y-pass: nx; x-pass: ny; 2-D: 1)
   int ncols;        // columns (= nh)
   int tile;         // 0: default tile width, 1: the alternative (chosen per plan by timing)
+  int wide;         // 1: keep the chosen width on small grids too (parity tests of the large-grid tiles)
 };
 
 struct Fields {

@@ -149,6 +149,44 @@ def nonlinear_3d(seed=3, n=512, nt=1500, pml=16):
     return kgrid, medium, source, sensor, (pml, pml, pml), (2.0, 2.0, 2.0)
 
 
+def nonlinear_3d_box(seed, shape, nt=300, long_pml=16, short_pml=4):
+    """The 512^3 workload's physics on an elongated box (parity cases of the 512-point kernels).
+
+    Gaussian-random c in [1450, 1650] and rho in [950, 1100] (correlation 8
+    cells), B/A = 6, power-law 0.5 dB/(MHz^1.1 cm); dx = 0.1 mm, dt = 0.3 dx /
+    1650.  The longest axis carries a 16-cell PML, the others 4.  An additive
+    8 x 8 plane source 24 cells along the long axis emits a 1.5 MHz, 3-cycle
+    tone burst of 2 MPa (shorter than nt, so the steps after it has ended run
+    too); a full cross-section sensor plane 60 cells further (or just inside the
+    far PML of a shorter box) records p and p_max.
+    """
+    rng = np.random.default_rng(seed)
+    shape = tuple(int(n) for n in shape)
+    ax = int(np.argmax(shape))
+    dx = 1e-4
+    c = _affine(_gaussian_filter_periodic(rng.standard_normal(shape), 8.0), 1450.0, 1650.0)
+    rho = _affine(_gaussian_filter_periodic(rng.standard_normal(shape), 8.0), 950.0, 1100.0)
+    kgrid = kWaveGrid(Vector(list(shape)), Vector([dx] * 3))
+    dt = 0.3 * dx / 1650.0
+    kgrid.setTime(int(nt), dt)
+    medium = kWaveMedium(sound_speed=c, density=rho, alpha_coeff=0.5, alpha_power=1.1, BonA=6.0)
+    src_sl = [slice(n // 2 - 4, n // 2 + 4) for n in shape]
+    src_sl[ax] = 24
+    mask = np.zeros(shape, dtype=bool)
+    mask[tuple(src_sl)] = True
+    source = kSource()
+    source.p_mask = mask
+    source.p = 2e6 * tone_burst(1 / dt, 1.5e6, 3)
+    sen_sl = [slice(None)] * 3
+    sen_sl[ax] = min(84, shape[ax] - long_pml - 2)
+    smask = np.zeros(shape, dtype=bool)
+    smask[tuple(sen_sl)] = True
+    sensor = kSensor(smask)
+    sensor.record = ["p", "p_max"]
+    pml = tuple(long_pml if i == ax else short_pml for i in range(3))
+    return kgrid, medium, source, sensor, pml, (2.0, 2.0, 2.0)
+
+
 SCAN_SIZES = ((256, 256), (384, 512), (512, 768), (640, 640))
 
 

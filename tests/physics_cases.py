@@ -150,7 +150,7 @@ def nonlinearity(runner, bona, *, ny=4, amp=1e6, f0=1e6):
 
 # ---------------------------------------------------------------------------
 def intensity(runner, *, ny=4, amp=1e5, f0=1e6, spp=30, ppw=30):
-    """Ix_avg of a steady plane wave divided by P^2 / (2 rho0 c0) (expect 1)."""
+    """Ix_avg and Iy_avg of a steady plane wave along x, in units of P^2 / (2 rho0 c0) (expect 1 and 0)."""
     from kwave_amd.kmedium import kWaveMedium
     from kwave_amd.ksensor import kSensor
     from kwave_amd.ksource import kSource
@@ -173,7 +173,8 @@ def intensity(runner, *, ny=4, amp=1e5, f0=1e6, spp=30, ppw=30):
     sen.record_start_index = start + 1
     r = runner(kg, med, src, sen, (20, 0), (2.0, 0.0))
     P2 = 2 * np.mean(np.asarray(r["p"][0], dtype=np.float64) ** 2)
-    return float(r["Ix_avg"][0]) / (P2 / (2 * RHO0 * C0)), float(r["Iy_avg"][0])
+    i_ref = P2 / (2 * RHO0 * C0)
+    return float(r["Ix_avg"][0]) / i_ref, float(r["Iy_avg"][0]) / i_ref
 
 
 # ---------------------------------------------------------------------------
