@@ -23,6 +23,7 @@ Prints ONE JSON line on stdout.  Everything else goes to stderr.
 """
 import argparse
 import json
+import math
 import os
 import platform
 import subprocess
@@ -72,26 +73,42 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(n=128, steps=5):
-    """FP64 oracle (numpy, single thread) on the same workload at n^3 for `steps` steps."""
+def cpu_baseline(n=256, n_target=512):
+    """FP64 oracle on the same workload at n^3: one warm-up and one timed step, single-threaded
+    numpy.fft (the reference's path), then the same with scipy.fft on every host core (at most 16).
+    The step time is scaled to n_target^3 by N log N (N voxels; the step is FFT-dominated), so the
+    rate scales by (N_target / N) / scale = log N / log N_target."""
     sys.path.insert(0, str(ROOT))
     from kwave_amd import workloads
     from oracle.kspace_ref import OracleSimulation
 
-    kg, med, src, sen, pml, alpha = workloads.nonlinear_3d(seed=3, n=n, nt=steps + 1, pml=max(2, 16 * n // 512))
-    sim = OracleSimulation(kg, med, src, sen, pml_size=pml, pml_alpha=alpha)
-    sim.step()  # warm-up
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    kg, med, src, sen, pml, alpha = workloads.nonlinear_3d(seed=3, n=n, nt=4, pml=max(2, 16 * n // 512))
+    scale = (n_target ** 3 * math.log(n_target ** 3)) / (n ** 3 * math.log(n ** 3))
+    out = {}
+    ncores = min(os.cpu_count() or 1, 16)
+    for workers in (1, ncores):
+        sim = OracleSimulation(kg, med, src, sen, pml_size=pml, pml_alpha=alpha, fft_workers=workers)
+        sim.step()  # warm-up
+        t0 = time.perf_counter()
         sim.step()
-    dt = time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        out[workers] = (n ** 3 / dt, dt)
+        del sim
+        log(f"cpu baseline: {n}^3 step {dt:.2f} s with {workers} FFT thread(s)")
+    rate1, dt1 = out[1]
+    ratem, dtm = out[ncores]
     return {
-        "value": n ** 3 * steps / dt,
+        "value": rate1 * (n_target ** 3 / n ** 3) / scale,
         "unit": "voxel-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"FP64 numpy oracle, same workload at {n}^3 for {steps} steps after 1 warm-up step "
-                  f"({dt:.1f} s, 1 thread of {os.cpu_count()} on '{cpu_model()}')",
+        "sample": f"FP64 numpy oracle (numpy.fft, as the reference), same 512^3 workload built at {n}^3: one timed step "
+                  f"after one warm-up ({dt1:.1f} s, {rate1:.3e} voxel-steps/s at {n}^3), scaled to {n_target}^3 by "
+                  f"N log N; 1 thread of {os.cpu_count()} on '{cpu_model()}'",
+        "measured_rate_at_sample": rate1,
+        "multicore": {"value": ratem * (n_target ** 3 / n ** 3) / scale, "cores": ncores, "measured_rate_at_sample": ratem,
+                      "sample": f"same, scipy.fft with {ncores} workers ({dtm:.1f} s per step at {n}^3); "
+                                f"element-wise NumPy work stays single-threaded"},
     }
 
 

@@ -98,7 +98,10 @@ int kw_abi_version(void);
 const char* kw_last_error(void);
 /* Number of visible HIP devices. */
 int kw_device_count(int* count);
-/* 1 if an axis of this length has a compiled transform, else 0. */
+/* 1 if an axis of this length has a transform, else 0: the lengths with fused
+ * kernels (2^k 16..1024, 3*2^k 24..768, 5*2^k 40..640, 7*2^k 56..448) and every
+ * other length from 2 to 4096 through the generic-length kernels (the
+ * reference's numpy.fft transforms any length, kspace_solver.py:702-705). */
 int kw_size_supported(int n);
 
 /* Replaces Simulation.__init__ + the grid part of setup(), kspace_solver.py:135-241. */

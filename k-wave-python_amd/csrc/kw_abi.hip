@@ -395,7 +395,7 @@ int kw_plan_create(const kw_plan_desc* d, kw_plan** out) {
   for (int i = 0; i < d->ndim; ++i) {
     if (d->n[i] < 2) return fail(KW_EINVAL, "every axis needs at least 2 points");
     if (!size_supported(d->n[i]))
-      return fail(KW_EUNSUPPORTED, "axis length " + std::to_string(d->n[i]) + " has no compiled transform");
+      return fail(KW_EUNSUPPORTED, "axis length " + std::to_string(d->n[i]) + " has no transform (lengths 2..4096 run)");
     if (!(d->spacing[i] > 0)) return fail(KW_EINVAL, "grid spacing must be positive");
   }
   int ndev = 0;
@@ -429,6 +429,14 @@ int kw_plan_create(const kw_plan_desc* d, kw_plan** out) {
   if (P->flags & (KW_POWERLAW_ABS | KW_POWERLAW_DISP | KW_NONLINEAR))
     if ((rc = P->mem.alloc(&P->fl.rhot, nf))) return cleanup(rc);
   if ((rc = ensure_spec(P, d->ndim - 1))) return cleanup(rc);
+  // generic-length family (kw_generic.hip): real temporaries for the unfused
+  // contiguous-axis updates, a stash spectrum for column passes with jobs
+  if (generic_z(P->g))
+    for (int i = 0; i < 4; ++i)
+      if ((rc = P->mem.alloc(&P->fl.tmp[i], nf))) return cleanup(rc);
+  for (int ax = 0; ax < d->ndim - 1; ++ax)
+    if (!compiled_size(d->n[ax]) && !P->fl.gstash)
+      if ((rc = P->mem.alloc(&P->fl.gstash, (size_t)P->nb * P->g.nspec))) return cleanup(rc);
   if ((rc = P->mem.alloc(&P->fl.tdev, P->nb))) return cleanup(rc);
   if ((rc = P->mem.alloc(&P->d_inst, P->nb))) return cleanup(rc);
   // operator tables
@@ -566,6 +574,7 @@ int kw_plan_set_source(kw_plan* P, int b, int which, int mode, int64_t npts, con
     if (hipMalloc(&tmp, sizeof(float2) * g.nspec) != hipSuccess) return fail(KW_ENOMEM, "pattern scratch allocation failed");
     Fields f{};
     f.spec[0] = tmp;
+    f.gstash = P->fl.gstash;  // generic-length x-pass: its forward-transform stash (one instance)
     const Inst* di = P->d_inst + b;
     const float2* twz = P->ops.tw[g.nd - 1];
     hipError_t e = launch_zfwd(g, 1, pat, 0, tmp, 0, twz, P->st);
@@ -842,23 +851,32 @@ int kw_plan_run_profiled(kw_plan* P, int64_t nsteps, double* kernel_ms, double* 
   int rc;
   if ((rc = sync_inst(P))) return rc;
   for (int k = 0; k < KK_COUNT; ++k) { kernel_ms[k] = 0.0; kernel_bytes[k] = 0.0; launches[k] = 0; }
-  P->prof = true;
-  P->ev_used = 0;
-  P->recs.clear();
   const int64_t qend = P->lean ? source_end(P) : -1;
-  for (int64_t s = 0; s < nsteps && rc == KW_OK; ++s) rc = enqueue_step(P, qend >= 0 && P->t_host >= 0 && P->t_host + s >= qend);
-  P->prof = false;
-  if (P->t_host >= 0) P->t_host += nsteps;
-  if (rc) return rc;
-  KW_HIP(hipStreamSynchronize(P->st));
-  for (const auto& r : P->recs) {
-    float ms = 0.f;
-    KW_HIP(hipEventElapsedTime(&ms, P->ev_pool[r.e0], P->ev_pool[r.e1]));
-    kernel_ms[r.kind] += ms;
-    kernel_bytes[r.kind] += r.bytes;
-    launches[r.kind] += 1;
+  // Steps run in chunks: after each chunk the stream is drained, the event pairs
+  // are read and the pool is reused, so the run holds at most one chunk's events
+  // (a few hundred) whatever its length.
+  constexpr int64_t kChunk = 8;
+  for (int64_t s0 = 0; s0 < nsteps && rc == KW_OK; s0 += kChunk) {
+    P->prof = true;
+    P->ev_used = 0;
+    P->recs.clear();
+    const int64_t s1 = std::min(nsteps, s0 + kChunk);
+    for (int64_t s = s0; s < s1 && rc == KW_OK; ++s) rc = enqueue_step(P, qend >= 0 && P->t_host >= 0 && P->t_host + s >= qend);
+    P->prof = false;
+    if (rc) break;
+    KW_HIP(hipStreamSynchronize(P->st));
+    for (const auto& r : P->recs) {
+      float ms = 0.f;
+      KW_HIP(hipEventElapsedTime(&ms, P->ev_pool[r.e0], P->ev_pool[r.e1]));
+      kernel_ms[r.kind] += ms;
+      kernel_bytes[r.kind] += r.bytes;
+      launches[r.kind] += 1;
+    }
   }
-  return KW_OK;
+  P->prof = false;
+  P->recs.clear();
+  if (P->t_host >= 0) P->t_host += nsteps;
+  return rc;
 }
 
 int kw_kernel_kind_count(void) { return KK_COUNT; }
@@ -956,6 +974,8 @@ int adjoint_step(kw_plan* P, const AdjFields& af, float* Zbuf, int t) {
 int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   if (!P || !adj || !grad_c || ckpt < 1) return fail(KW_EINVAL, "NULL argument or checkpoint interval < 1");
   if (P->flags) return fail(KW_EUNSUPPORTED, "adjoint: linear lossless plans only");
+  if (generic_z(P->g) || P->fl.gstash)
+    return fail(KW_EUNSUPPORTED, "adjoint: grids whose axis lengths (or odd row count) need the generic transforms are not differentiated");
   const Geom& g = P->g;
   const int D = g.nd;
   const int nb = P->nb;

@@ -1,6 +1,8 @@
 // Size dispatch: each launcher is defined once per size family (f_g0 .. f_g3,
 // see kw_kernels.hpp); the entry points below hand a call to the family that
-// holds the line length.
+// holds the line length, and every other grid to the generic-length family
+// (kw_generic.hip): contiguous-axis work whose length is not compiled or whose
+// row count is odd, column passes whose length is not compiled.
 #include "kw_launch.hpp"
 
 namespace kw {
@@ -30,27 +32,35 @@ KW_DECL4(hipError_t, launch_adj_mom, const Geom&, int, const Fields&, const AdjF
 KW_DECL4(hipError_t, launch_adj_sum, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, hipStream_t)
 
 hipError_t launch_zfwd(const Geom& g, int nb, const float* f, int64_t fs, float2* spec, int64_t ss, const float2* tw, hipStream_t st) {
+  if (generic_z(g)) return launch_zfwd_gen(g, nb, f, fs, spec, ss, tw, st);
   KW_TRY4(launch_zfwd, g, nb, f, fs, spec, ss, tw, st);
 }
 hipError_t launch_zinv(const Geom& g, int nb, const float2* spec, int64_t ss, float* f, int64_t fs, const float2* tw, float scale, hipStream_t st) {
+  if (generic_z(g)) return launch_zinv_gen(g, nb, spec, ss, f, fs, tw, scale, st);
   KW_TRY4(launch_zinv, g, nb, spec, ss, f, fs, tw, scale, st);
 }
 hipError_t launch_init_u(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st) {
+  if (generic_z(g)) return launch_init_u_gen(g, nb, fl, inst, tw, st);
   KW_TRY4(launch_init_u, g, nb, fl, inst, tw, st);
 }
 hipError_t launch_src_build(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int which, hipStream_t st) {
+  if (generic_z(g)) return launch_src_build_gen(g, nb, fl, inst, tw, which, st);
   KW_TRY4(launch_src_build, g, nb, fl, inst, tw, which, st);
 }
 hipError_t launch_momentum(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st) {
+  if (generic_z(g)) return launch_momentum_gen(g, nb, fl, inst, tw, st);
   KW_TRY4(launch_momentum, g, nb, fl, inst, tw, st);
 }
 hipError_t launch_mass(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int flags, hipStream_t st, bool sreg) {
+  if (generic_z(g)) return launch_mass_gen(g, nb, fl, inst, tw, flags, st);
   KW_TRY4(launch_mass, g, nb, fl, inst, tw, flags, st, sreg);
 }
 hipError_t launch_eos(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int flags, hipStream_t st) {
+  if (generic_z(g)) return launch_eos_gen(g, nb, fl, inst, tw, flags, st);
   KW_TRY4(launch_eos, g, nb, fl, inst, tw, flags, st);
 }
 hipError_t launch_col(const Geom& g, int nb, const Fields& fl, const Ops& ops, const Inst* inst, const ColArgs& a, int nline, hipStream_t st) {
+  if (!compiled_size(nline)) return launch_col_gen(g, nb, fl, ops, inst, a, nline, st);
   KW_TRY4(launch_col, g, nb, fl, ops, inst, a, nline, st);
 }
 

@@ -6,7 +6,22 @@
 #include "kw_types.hpp"
 
 namespace kw {
+// Any axis length with a transform: the compiled (fused) lengths, plus 2..4096 through the
+// generic family of kw_generic.hip.
 bool size_supported(int n);
+bool compiled_size(int n);
+// The contiguous-axis work of this grid goes to the generic family (length not
+// compiled, or an odd number of rows).
+bool generic_z(const Geom& g);
+// Generic-length family: same contracts as the launchers below (kw_generic.hip).
+hipError_t launch_zfwd_gen(const Geom& g, int nb, const float* f, int64_t fs, float2* spec, int64_t ss, const float2* tw, hipStream_t st);
+hipError_t launch_zinv_gen(const Geom& g, int nb, const float2* spec, int64_t ss, float* f, int64_t fs, const float2* tw, float scale, hipStream_t st);
+hipError_t launch_momentum_gen(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st);
+hipError_t launch_mass_gen(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int flags, hipStream_t st);
+hipError_t launch_eos_gen(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int flags, hipStream_t st);
+hipError_t launch_init_u_gen(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st);
+hipError_t launch_src_build_gen(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int which, hipStream_t st);
+hipError_t launch_col_gen(const Geom& g, int nb, const Fields& fl, const Ops& ops, const Inst* inst, const ColArgs& a, int nline, hipStream_t st);
 hipError_t launch_zfwd(const Geom& g, int nb, const float* f, int64_t fs, float2* spec, int64_t ss, const float2* tw, hipStream_t st);
 hipError_t launch_zinv(const Geom& g, int nb, const float2* spec, int64_t ss, float* f, int64_t fs, const float2* tw, float scale, hipStream_t st);
 hipError_t launch_momentum(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st);

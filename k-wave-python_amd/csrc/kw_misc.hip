@@ -109,7 +109,6 @@ hipError_t launch_adj_srcgrad(const Geom& g, int nb, const AdjFields& af, const 
   return hipGetLastError();
 }
 
-bool size_supported(int n) { return size_supported_impl(n); }
 
 hipError_t launch_gather(const Geom& g, int nb, int max_sens, const Fields& fl, const Inst* inst, hipStream_t st) {
   if (max_sens <= 0) return hipSuccess;

@@ -112,6 +112,10 @@ struct Fields {
   float2* spec[kMaxBufs];
   float* uc[3];     // velocity on the pressure grid (only when it is recorded)
   int* tdev;        // per-instance step counter
+  // generic-length family only (kw_generic.hip): real temporaries of the unfused
+  // contiguous-axis updates, and the stash of a column pass's forward transform
+  float* tmp[4];
+  float2* gstash;
 };
 
 // State of the reverse-time sweep of kw_plan_adjoint_c (all [nb][ntot] unless noted).

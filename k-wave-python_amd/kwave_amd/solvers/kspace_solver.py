@@ -141,7 +141,8 @@ class Simulation:
         self.num_recorded_time_points = st.Nt - st.rec_start
         self._plan = DevicePlan([st], device=self._device)
         # the device applies the t = 0 initial-value step during plan set-up;
-        # the host counter starts at 0 like the reference and catches up in step()
+        # the host counter starts at 0 like the reference and catches up in step(),
+        # and the state properties show the reference's zeros until then
         self._t = 0
         self._is_setup = True
         return self
@@ -172,17 +173,25 @@ class Simulation:
         return collect_results(self._plan, 0, self._setup, self._dtype)
 
     # -- state inspection (reads device memory) ------------------------------
+    # Before the first step() the reference's fields are the zeros of
+    # _setup_fields (kspace_solver.py:651-654); the device plan has already
+    # applied the t = 0 initial-value override by then, so report zeros.
+    def _state(self, which):
+        if self._t == 0:
+            return np.zeros(self.grid_shape, dtype=self._dtype)
+        return self._plan.field(0, which).astype(self._dtype)
+
     @property
     def p(self):
-        return self._plan.field(0, _abi.F_P).astype(self._dtype)
+        return self._state(_abi.F_P)
 
     @property
     def u(self):
-        return [self._plan.field(0, _abi.F_UX + i).astype(self._dtype) for i in range(self.ndim)]
+        return [self._state(_abi.F_UX + i) for i in range(self.ndim)]
 
     @property
     def rho_split(self):
-        return [self._plan.field(0, _abi.F_RHOX + i).astype(self._dtype) for i in range(self.ndim)]
+        return [self._state(_abi.F_RHOX + i) for i in range(self.ndim)]
 
     def close(self):
         if self._plan is not None:

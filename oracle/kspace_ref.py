@@ -142,14 +142,26 @@ def _expand(val, shape, name):
 # ---------------------------------------------------------------------------
 
 class OracleSimulation:
-    """FP64 restatement of ``Simulation`` (kwave/solvers/kspace_solver.py:122-778)."""
+    """FP64 restatement of ``Simulation`` (kwave/solvers/kspace_solver.py:122-778).
+
+    ``fft_workers`` > 1 runs the N-D transforms through ``scipy.fft`` with that
+    many threads (the same pocketfft algorithm as numpy.fft; a multi-core timing
+    variant of the CPU baseline); the default is numpy.fft, as the reference.
+    """
 
     def __init__(self, kgrid, medium, source, sensor, *, use_sg=True, use_kspace=True,
-                 smooth_p0=True, pml_size=None, pml_alpha=None):
+                 smooth_p0=True, pml_size=None, pml_alpha=None, fft_workers=1):
         self.kgrid, self.medium, self.source, self.sensor = kgrid, medium, source, sensor
         self.use_sg, self.use_kspace, self.smooth_p0 = use_sg, use_kspace, smooth_p0
         self._pml_size = pml_size
         self._pml_alpha = pml_alpha
+        if fft_workers > 1:
+            import scipy.fft as sfft
+
+            self._fftn = lambda a: sfft.fftn(a, workers=fft_workers)
+            self._ifftn = lambda a: sfft.ifftn(a, workers=fft_workers)
+        else:
+            self._fftn, self._ifftn = np.fft.fftn, np.fft.ifftn
         self.t = 0
         self._setup()
 
@@ -445,7 +457,7 @@ class OracleSimulation:
     # -- step: kspace_solver.py:691-747 ---------------------------------------
     def _diff(self, f, op):
         """kspace_solver.py:781-796."""
-        return np.real(np.fft.ifftn(op * np.fft.fftn(f)))
+        return np.real(self._ifftn(op * self._fftn(f)))
 
     def _mass_factor(self):
         """(2 sum(rho_i) + rho0) / rho0 of the nonlinear mass equation, 1 when linear;
@@ -456,9 +468,9 @@ class OracleSimulation:
         if self.t >= self.Nt:
             return self
         nd = self.ndim
-        P = np.fft.fftn(self.p)
+        P = self._fftn(self.p)
         for i in range(nd):
-            g = np.real(np.fft.ifftn(self.op_grad[i] * P))
+            g = np.real(self._ifftn(self.op_grad[i] * P))
             self.u[i] = self.pml_sg[i] * (self.pml_sg[i] * self.u[i] - self.dt_rho0[i] * g)
             self.u[i] = self.src_u[i](self.t, self.u[i])
         nl = self._mass_factor()
