@@ -1,7 +1,9 @@
-"""Unified entry point (mirrors kwave/kspaceFirstOrder.py:16-350), python backend on the GPU.
+"""Unified entry point (mirrors kwave/kspaceFirstOrder.py:16-350) on the GPU.
 
 PML-outside expansion and stripping, dtype resolution, ``pml_size="auto"``,
-validation and p0 smoothing after expansion, then the device ``Simulation``.
+validation and p0 smoothing after expansion, then the device ``Simulation``
+(backend "python") or the C++ binaries' HDF5 file route run on the GPU
+(backend "cpp", solvers/cpp_simulation.py).
 """
 import copy
 import warnings
@@ -94,15 +96,15 @@ def kspaceFirstOrder(kgrid, medium, source, sensor=None, *, pml_size=20, pml_alp
                      binary_path=None) -> dict:
     """Run a k-Wave simulation on the MI355X (kspaceFirstOrder.py:119-326).
 
-    ``backend`` must be "python" (the native solver path); ``device`` "gpu"
-    or "cpu" both run the HIP solver; ``device_num`` selects the HIP device.
+    ``backend`` "python" is the native solver path; "cpp" writes the k-Wave
+    C++ binaries' HDF5 input file and runs it on the MI355X (``save_only``
+    writes the file only).  ``device`` "gpu" or "cpu" both run the HIP solver;
+    ``device_num`` selects the HIP device.
     """
     if device not in ("cpu", "gpu"):
         raise ValueError(f"device must be 'cpu' or 'gpu', got {device!r}")
     if backend not in ("python", "cpp"):
         raise ValueError(f"Unknown backend: {backend!r}. Use 'python' or 'cpp'.")
-    if backend == "cpp":
-        raise NotImplementedError("backend='cpp' (the external C++ binaries) is not part of this package")
     dtype = _resolve_dtype(dtype)
     if isinstance(pml_size, str) and pml_size.lower() == "auto":
         pml_size = tuple(int(x) for x in get_optimal_pml_size(kgrid))
@@ -123,6 +125,16 @@ def kspaceFirstOrder(kgrid, medium, source, sensor=None, *, pml_size=20, pml_alp
 
     if kgrid.dim == 1:
         raise NotImplementedError("1-D grids are not available on the GPU path")
+    if backend == "cpp":
+        result = _run_cpp_route(kgrid, medium, source, sensor, pml_size, pml_alpha, use_sg, use_kspace, dtype, save_only,
+                                data_path, device_num)
+        if save_only:
+            if not pml_inside:
+                result["pml_size"] = pml_size
+            return result
+        if not pml_inside:
+            result = _strip_pml(result, pml_size, kgrid.dim, suffixes=("_final",) + _FULL_GRID_SUFFIXES)
+        return result
     with warnings.catch_warnings():
         warnings.simplefilter("ignore", UserWarning)
         result = Simulation(kgrid, medium, source, sensor, device=device_num if device_num is not None else "gpu",
@@ -131,6 +143,51 @@ def kspaceFirstOrder(kgrid, medium, source, sensor=None, *, pml_size=20, pml_alp
     if not pml_inside:
         result = _strip_pml(result, pml_size, kgrid.dim)
     return result
+
+
+_CPP_UNSUPPORTED_ALPHA_MODES = ("no_absorption", "no_dispersion", "stokes")
+_DISPERSION_SINGULAR = (0.95, 1.05)
+
+
+def _run_cpp_route(kgrid, medium, source, sensor, pml_size, pml_alpha, use_sg, use_kspace, dtype, save_only, data_path,
+                   device_num):
+    """backend="cpp": the C++ binaries' file route (kspaceFirstOrder.py:271-319), run on the MI355X
+    (solvers/cpp_simulation.py).  Same checks and warnings as the reference's route."""
+    from .solvers.cpp_simulation import CppSimulation
+    from .utils.conversion import cart2grid
+
+    mode = getattr(medium, "alpha_mode", None)
+    mode = getattr(mode, "value", mode)
+    if mode in _CPP_UNSUPPORTED_ALPHA_MODES:
+        raise ValueError(f"medium.alpha_mode={mode!r} is not supported by the C++ backend: its HDF5 input format does not "
+                         "carry alpha_mode, so full power-law absorption and dispersion would apply. Use backend='python'.")
+    a, y = getattr(medium, "alpha_coeff", None), getattr(medium, "alpha_power", None)
+    if a is not None and y is not None and np.any(np.asarray(a) > 0):
+        yy = np.asarray(y, dtype=float)
+        if np.any((yy >= _DISPERSION_SINGULAR[0]) & (yy <= _DISPERSION_SINGULAR[1])):
+            warnings.warn(f"medium.alpha_power in the dispersion-singular range {list(_DISPERSION_SINGULAR)} on the C++ backend: "
+                          "tan(pi y / 2) overflows in float32 (issue 664); use backend='python' with "
+                          "alpha_mode='no_dispersion'.", stacklevel=3)
+    if dtype is not np.float64:
+        warnings.warn(f"dtype={np.dtype(dtype).name!r} has no effect with backend='cpp'.", stacklevel=3)
+    if not use_kspace:
+        warnings.warn("use_kspace=False has no effect with backend='cpp'; the k-space correction always applies.", stacklevel=3)
+    if sensor is not None and getattr(sensor, "record", None) is not None:
+        warnings.warn("sensor.record is not yet supported for backend='cpp'; the pressure time series is recorded.",
+                      stacklevel=3)
+    if sensor is not None and getattr(sensor, "record_start_index", 1) != 1:
+        warnings.warn("sensor.record_start_index is not yet supported for backend='cpp'; recording starts at the first step.",
+                      stacklevel=3)
+    if sensor is not None and getattr(sensor, "mask", None) is not None and _is_cartesian_mask(sensor.mask, kgrid.dim):
+        sensor = copy.copy(sensor)
+        sensor.mask = cart2grid(kgrid, np.asarray(sensor.mask), order="C")[0] != 0
+    sim = CppSimulation(kgrid, medium, source, sensor, pml_size=pml_size, pml_alpha=pml_alpha, use_sg=use_sg)
+    if save_only:
+        if data_path is None:
+            raise ValueError("data_path must be provided when save_only=True (the HDF5 files must persist for cluster submission).")
+        inp, outp = sim.prepare(data_path=data_path)
+        return {"input_file": inp, "output_file": outp}
+    return sim.run(device_num=device_num, data_path=data_path)
 
 
 def reshape_to_grid(data, grid_shape):

@@ -181,8 +181,15 @@ class InstanceSetup:
 
 
 def build_instance(kgrid, medium, source, sensor, *, use_sg=True, use_kspace=True, smooth_p0=True,
-                   pml_size=None, pml_alpha=None):
-    """Host set-up of one instance (kspace_solver.py:205-689), float64 throughout."""
+                   pml_size=None, pml_alpha=None, source_scaled=False, rho0_sg=None):
+    """Host set-up of one instance (kspace_solver.py:205-689), float64 throughout.
+
+    ``source_scaled``: the time-varying source signals are already in the units the
+    time loop injects (the k-Wave binaries' input files hold them so): no
+    2 dt/(D c0 d) / 1/(D c0^2) / 2 c0 dt/d scaling, and a pressure source adds
+    (or sets) its value in every split density.  ``rho0_sg``: staggered-grid
+    densities per axis (scalars or grids) to use instead of the midpoint average.
+    """
     from .. import _abi
 
     shape, spacing = grid_shape_and_spacing(kgrid)
@@ -233,7 +240,12 @@ def build_instance(kgrid, medium, source, sensor, *, use_sg=True, use_kspace=Tru
             pml_sg.append(get_pml(shape[ax], spacing[ax], dt, c_ref, size, alpha, staggered=True).ravel())
 
     # --- staggered density (kspace_solver.py:656-663) ---
-    if r_scalar:
+    if rho0_sg is not None:
+        dtr0 = []
+        for a in range(nd):
+            sg = np.asarray(rho0_sg[a], dtype=np.float64)
+            dtr0.append(dt / (float(sg.ravel()[0]) if sg.size == 1 else expand_to_grid(sg, shape, "rho0_sg")))
+    elif r_scalar:
         dtr0 = [dt / r_val] * nd
     else:
         dtr0 = [dt / (stagger(r_val, a) if use_sg else r_val) for a in range(nd)]
@@ -304,7 +316,9 @@ def build_instance(kgrid, medium, source, sensor, *, use_sg=True, use_kspace=Tru
     src = source
     p_mask, p_sig = getattr(src, "p_mask", 0), getattr(src, "p", 0)
     p_mode = getattr(src, "p_mode", None) or "additive"
-    if is_enabled(p_mask) and is_enabled(p_sig):
+    if is_enabled(p_mask) and is_enabled(p_sig) and source_scaled:
+        st.src_p = term(p_mask, p_sig, p_mode, lambda c: np.ones_like(c), [1.0] * nd)
+    elif is_enabled(p_mask) and is_enabled(p_sig):
         if p_mode == "dirichlet":
             st.src_p = term(p_mask, p_sig, p_mode, lambda c: 1.0 / (nd * c ** 2), [1.0] * nd)
         else:
@@ -315,7 +329,7 @@ def build_instance(kgrid, medium, source, sensor, *, use_sg=True, use_kspace=Tru
     for i, name in enumerate(["ux", "uy", "uz"][:nd]):
         sig = getattr(src, name, 0)
         if is_enabled(u_mask) and is_enabled(sig):
-            if u_mode == "dirichlet":
+            if u_mode == "dirichlet" or source_scaled:
                 fn = lambda c: np.ones_like(c)  # noqa: E731
             else:
                 fn = lambda c, _d=spacing[i]: 2 * c * dt / _d  # noqa: E731

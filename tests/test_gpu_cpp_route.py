@@ -1,0 +1,97 @@
+"""The C++ binaries' file route on the GPU: kspaceFirstOrder(backend="cpp") writes the HDF5 input
+file, the runner executes it on the MI355X and the output comes back in C order.  The result must
+equal the native route (backend="python") on the same problem: both run the same kernels; the file
+route's inputs pass through float32 storage (c0, rho0_sg, dt, the pre-scaled source signals), so the
+bar is rel-L2 < 1e-5.  Also the runner's command line on a prepared file (-i / -o and output flags)."""
+import copy
+import subprocess
+import sys
+import warnings
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import _grid, _hetero, _p0
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def _case(kind):
+    from kwave_amd.kmedium import kWaveMedium
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.ksource import kSource
+
+    shape = (48, 40, 36) if kind == "ivp_3d" else (64, 72)
+    kg = _grid(shape, nt=120)
+    c, r = _hetero(shape, 11)
+    med = kWaveMedium(sound_speed=c, density=r, alpha_coeff=0.5, alpha_power=1.4)
+    src = kSource()
+    t = np.arange(60) * float(kg.dt)
+    base = np.sin(2 * np.pi * 1e6 * t) * np.hanning(60)
+    m = np.zeros(shape, dtype=bool)
+    if kind.startswith("ivp"):
+        src.p0 = 1e3 * _p0(shape, 12)
+    elif kind == "velocity":
+        m[12, 20:30] = True
+        src.u_mask = m
+        src.ux = 1e-2 * base[None, :]
+        src.uy = np.stack([-5e-3 * np.roll(base, k) for k in range(int(m.sum()))])
+    else:
+        m[12, 20:36] = True
+        src.p_mask = m
+        npt = int(m.sum())
+        src.p = 1e3 * (base[None, :] if kind == "additive" else np.stack([np.roll(base, k) for k in range(npt)]))
+        src.p_mode = "dirichlet" if kind == "dirichlet" else "additive"
+    sm = np.zeros(shape, dtype=bool)
+    sm[tuple(slice(s // 4, 3 * s // 4, 3) for s in shape)] = True
+    return kg, med, src, kSensor(sm)
+
+
+@pytest.mark.parametrize("kind", ["ivp", "ivp_3d", "additive", "additive_many", "dirichlet", "velocity"])
+def test_cpp_route_equals_native_route(kind, tmp_path):
+    from kwave_amd.kspaceFirstOrder import kspaceFirstOrder
+
+    kg, med, src, sen = _case(kind)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        native = kspaceFirstOrder(kg, med, copy.deepcopy(src), sen, pml_size=8)
+        filed = kspaceFirstOrder(kg, med, copy.deepcopy(src), sen, pml_size=8, backend="cpp", data_path=str(tmp_path))
+    assert filed["p"].shape == native["p"].shape
+    rel = _rel(filed["p"], native["p"])
+    print(f"{kind}: file route vs native route p rel-L2 {rel:.2e}")
+    assert rel < 1e-5
+
+
+def test_runner_command_line(tmp_path):
+    from kwave_amd.solvers.cpp_simulation import CppSimulation
+    from kwave_amd.solvers.kspace_solver import Simulation
+
+    kg, med, src, sen = _case("additive_many")
+    sim = CppSimulation(kg, med, copy.deepcopy(src), sen, pml_size=(8, 8), pml_alpha=(2.0, 2.0))
+    inp, outp = sim.prepare(str(tmp_path))
+    r = subprocess.run([sys.executable, "-m", "kwave_amd.solvers.cpp_runner", "-i", inp, "-o", outp, "--p_raw", "--p_max",
+                        "--p_final", "--u_non_staggered_raw"], cwd=str(ROOT / "k-wave-python_amd"), capture_output=True,
+                       text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = sim._fix_output_order(sim._parse_output(outp))
+    assert set(got) == {"p", "p_max", "p_final", "ux_non_staggered", "uy_non_staggered"}
+    sen2 = copy.copy(sen)
+    sen2.record = ["p", "p_max", "p_final", "u"]
+    want = Simulation(kg, med, copy.deepcopy(src), sen2, pml_size=(8, 8), pml_alpha=(2.0, 2.0)).run()
+    assert got["p_final"].shape == tuple(kg.N)
+    pairs = [("p", want["p"]), ("p_max", want["p_max"]), ("p_final", None), ("ux_non_staggered", want["ux"]),
+             ("uy_non_staggered", want["uy"])]
+    for k, w in pairs:
+        if k == "p_final":
+            g, w = got[k][8:-8, 8:-8], want["p_final"]
+        else:
+            g = got[k]
+        rel = _rel(g, w)
+        print(f"runner {k}: rel-L2 {rel:.2e}")
+        assert rel < 1e-5, k
