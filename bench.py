@@ -175,10 +175,12 @@ def small_workloads():
 
 
 def surrogate_leg():
-    """Learned surrogate of the 128^2 problem: train on solver data, held-out error, speed vs the solver."""
+    """Learned surrogate of the 128^2 problem (4 layers, 32 channels, 16 x 16 modes): train on 20k
+    solver samples, calibrate the per-map fallback on 3k more, held-out error on 1k before and
+    after routing, speed vs the batched solver with and without the fallback."""
     from kwave_amd import surrogate
 
-    res, _ = surrogate.benchmark(n_train=4096, n_test=512, epochs=80, log=log)
+    res, _ = surrogate.benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=24, log=log)
     return res
 
 

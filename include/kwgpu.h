@@ -135,6 +135,21 @@ int kw_plan_run(kw_plan* plan, int64_t nsteps);
  * calling Simulation.run() per instance (kspace_solver.py:749-761). */
 int kw_plan_launch(kw_plan* plan, int64_t nsteps);
 int kw_plan_sync(kw_plan* plan);
+
+/* Several initialised plans on one device -- typically the grid-shape groups of
+ * an uneven sweep, each a batch of same-shape instances -- advanced together:
+ * every block of steps is one captured graph in which each plan's step sequence
+ * is a parallel branch, launched with a single graph launch.  kw_batch_run runs
+ * every plan until each of its instances has reached its own Nt (finished plans
+ * leave the graph; a plan switches to its lean step form once its filtered
+ * sources have ended).  Results equal those of running the plans one by one.
+ * The plans stay owned by the caller and must outlive the batch; no other call
+ * may use them during kw_batch_run.  The batched counterpart of calling
+ * Simulation.run() once per instance (kspace_solver.py:749-778). */
+typedef struct kw_batch kw_batch;
+int kw_batch_create(kw_plan* const* plans, int n, kw_batch** out);
+int kw_batch_run(kw_batch* batch);
+int kw_batch_destroy(kw_batch* batch);
 /* Like kw_plan_run, without graph capture, bracketing every kernel launch with
  * HIP events on the plan stream.  Per kernel kind k (< kw_kernel_kind_count())
  * it returns the summed device time [ms], the algorithmic bytes moved and the

@@ -7,6 +7,7 @@
 #include <complex>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -103,6 +104,8 @@ struct kw_plan {
   int col_tile[2] = {0, 0};  // tile variant of the FFT-only and the spectral-job column passes
   bool tuned = false;
   bool col_wide = false;  // KW_COL_WIDE=1: no narrow-tile fallback on small grids
+  int col_persist = 3;    // KW_COL_PERSIST: column passes that loop over tiles (ColArgs::persist)
+  int col_grid = 0;       // KW_COL_GRID: workgroups per instance of looping passes (0: resident grid)
   bool inst_dirty = true;
   // per-kernel timing (kw_plan_run_profiled)
   bool prof = false;
@@ -204,6 +207,8 @@ int ypass(kw_plan* P, int mode, const std::vector<int>& bufs) {
   ColArgs a = col_args(P->g, 1, mode);
   a.tile = P->col_tile[0];
   a.wide = P->col_wide;
+  a.persist = P->col_persist;
+  a.grid = P->col_grid;
   a.nbuf = (int)bufs.size();
   for (size_t i = 0; i < bufs.size(); ++i) a.bufs[i] = bufs[i];
   const double spec_bytes = 8.0 * P->nb * (double)P->g.rows * P->g.nh;
@@ -215,6 +220,8 @@ int xpass(kw_plan* P, const std::vector<XJob>& jobs, int kind) {
   ColArgs a = col_args(P->g, 0, 2);
   a.tile = P->col_tile[1];
   a.wide = P->col_wide;
+  a.persist = P->col_persist;
+  a.grid = P->col_grid;
   a.njob = (int)jobs.size();
   int nin = 0, last = -1;
   for (size_t i = 0; i < jobs.size(); ++i) {
@@ -415,6 +422,10 @@ int kw_plan_create(const kw_plan_desc* d, kw_plan** out) {
   {
     const char* w = std::getenv("KW_COL_WIDE");
     P->col_wide = w && std::atoi(w) != 0;
+    const char* pe = std::getenv("KW_COL_PERSIST");
+    if (pe && pe[0] >= '0' && pe[0] <= '3') P->col_persist = pe[0] - '0';
+    const char* ge = std::getenv("KW_COL_GRID");
+    P->col_grid = ge ? std::max(0, std::atoi(ge)) : 0;
   }
   int rc = KW_OK;
   auto cleanup = [&](int code) { delete P; return code; };
@@ -717,6 +728,8 @@ int autotune_tiles(kw_plan* P) {
         if (D == 3) return ypass(P, 0, bufs);
         ColArgs a = col_args(g, 0, 0);
         a.tile = tile;
+        a.persist = P->col_persist;
+        a.grid = P->col_grid;
         a.nbuf = D;
         for (int i = 0; i < D; ++i) a.bufs[i] = i;
         KW_LAUNCH(launch_col(g, P->nb, P->fl, P->ops, P->d_inst, a, g.n[0], P->st));
@@ -1137,6 +1150,145 @@ int kw_plan_destroy(kw_plan* P) {
   (void)hipSetDevice(P->dev);
   if (P->st) (void)hipStreamSynchronize(P->st);
   delete P;
+  return KW_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// several plans as one captured step graph (kw_batch_*)
+// ---------------------------------------------------------------------------
+struct kw_batch {
+  std::vector<kw_plan*> plans;
+  int dev = 0;
+  hipStream_t st = nullptr;  // the stream graphs are launched on
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  std::map<std::string, hipGraphExec_t> graphs;  // key: block length and each plan's form
+  int graph_k = 8;
+  ~kw_batch() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (hipEvent_t e : join) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+// Steps each plan still has to run: its instances' largest Nt minus the common step index.
+int64_t plan_remaining(const kw_plan* P) {
+  int nt = 0;
+  for (const Inst& I : P->h_inst) nt = std::max(nt, I.nt);
+  return std::max<int64_t>(0, nt - P->t_host);
+}
+
+// One graph for k steps of every plan marked in `form` ('-': idle, 'f': full step, 'l': lean
+// step): each plan's steps are a branch forked from and joined into the batch stream.
+int batch_graph(kw_batch* B, int k, const std::string& form, hipGraphExec_t* out) {
+  const std::string key = std::to_string(k) + ":" + form;
+  auto it = B->graphs.find(key);
+  if (it != B->graphs.end()) { *out = it->second; return KW_OK; }
+  KW_HIP(hipStreamBeginCapture(B->st, hipStreamCaptureModeThreadLocal));
+  int rc = KW_OK;
+  hipError_t e = hipEventRecord(B->fork, B->st);
+  for (size_t i = 0; i < B->plans.size() && rc == KW_OK && e == hipSuccess; ++i) {
+    if (form[i] == '-') continue;
+    kw_plan* P = B->plans[i];
+    e = hipStreamWaitEvent(P->st, B->fork, 0);
+    for (int s = 0; s < k && rc == KW_OK && e == hipSuccess; ++s) rc = enqueue_step(P, form[i] == 'l');
+    if (rc == KW_OK && e == hipSuccess) e = hipEventRecord(B->join[i], P->st);
+    if (rc == KW_OK && e == hipSuccess) e = hipStreamWaitEvent(B->st, B->join[i], 0);
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(B->st, &graph);
+  if (rc != KW_OK || e != hipSuccess || ec != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != KW_OK) return rc;
+    return fail(KW_EHIP, std::string("batch graph capture: ") + hipGetErrorString(e != hipSuccess ? e : ec));
+  }
+  hipGraphExec_t ex = nullptr;
+  e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(KW_EHIP, std::string("batch graph instantiate: ") + hipGetErrorString(e));
+  B->graphs[key] = ex;
+  *out = ex;
+  return KW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kw_batch_create(kw_plan* const* plans, int n, kw_batch** out) {
+  if (!plans || n < 1 || !out) return fail(KW_EINVAL, "batch needs at least one plan");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    if (!plans[i]) return fail(KW_EINVAL, "NULL plan in batch");
+    if (plans[i]->dev != plans[0]->dev) return fail(KW_EINVAL, "every plan of a batch must be on one device");
+    for (int j = 0; j < i; ++j)
+      if (plans[j] == plans[i]) return fail(KW_EINVAL, "a plan appears twice in the batch");
+  }
+  KW_HIP(hipSetDevice(plans[0]->dev));
+  auto* B = new kw_batch();
+  B->dev = plans[0]->dev;
+  B->plans.assign(plans, plans + n);
+  B->join.assign(n, nullptr);
+  auto cleanup = [&](int code) { delete B; return code; };
+  if (hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking) != hipSuccess) return cleanup(fail(KW_EHIP, "batch stream creation failed"));
+  if (hipEventCreateWithFlags(&B->fork, hipEventDisableTiming) != hipSuccess) return cleanup(fail(KW_EHIP, "batch event creation failed"));
+  for (int i = 0; i < n; ++i)
+    if (hipEventCreateWithFlags(&B->join[i], hipEventDisableTiming) != hipSuccess) return cleanup(fail(KW_EHIP, "batch event creation failed"));
+  *out = B;
+  return KW_OK;
+}
+
+int kw_batch_run(kw_batch* B) {
+  if (!B) return fail(KW_EINVAL, "NULL batch");
+  KW_HIP(hipSetDevice(B->dev));
+  const size_t n = B->plans.size();
+  std::vector<int64_t> rem(n), qend(n);
+  int64_t total = 0;
+  int rc;
+  for (size_t i = 0; i < n; ++i) {
+    kw_plan* P = B->plans[i];
+    if (P->t_host < 0) return fail(KW_EINVAL, "every plan of a batch must be initialised (kw_plan_init) first");
+    if ((rc = sync_inst(P))) return rc;
+    KW_HIP(hipStreamSynchronize(P->st));  // work enqueued on the plan outside the batch comes first
+    rem[i] = plan_remaining(P);
+    qend[i] = P->lean ? source_end(P) : -1;
+    total = std::max(total, rem[i]);
+  }
+  std::vector<int64_t> done(n, 0);  // steps enqueued per plan
+  for (int64_t s = 0; s < total;) {
+    // block length: graph_k steps unless a plan switches to its lean form inside the block
+    int k = (int)std::min<int64_t>(B->graph_k, total - s);
+    for (size_t i = 0; i < n; ++i) {
+      const kw_plan* P = B->plans[i];
+      if (s >= rem[i] || qend[i] < 0) continue;
+      const int64_t t = P->t_host + s;
+      if (t < qend[i] && t + k > qend[i]) k = (int)(qend[i] - t);
+    }
+    std::string form(n, '-');
+    for (size_t i = 0; i < n; ++i)
+      if (s < rem[i]) form[i] = (qend[i] >= 0 && B->plans[i]->t_host + s >= qend[i]) ? 'l' : 'f';
+    hipGraphExec_t g = nullptr;
+    if ((rc = batch_graph(B, k, form, &g))) return rc;
+    KW_HIP(hipGraphLaunch(g, B->st));
+    for (size_t i = 0; i < n; ++i)
+      if (form[i] != '-') done[i] += k;
+    s += k;
+  }
+  KW_HIP(hipStreamSynchronize(B->st));
+  for (size_t i = 0; i < n; ++i) B->plans[i]->t_host += done[i];
+  return KW_OK;
+}
+
+int kw_batch_destroy(kw_batch* B) {
+  if (!B) return KW_OK;
+  (void)hipSetDevice(B->dev);
+  if (B->st) (void)hipStreamSynchronize(B->st);
+  delete B;
   return KW_OK;
 }
 

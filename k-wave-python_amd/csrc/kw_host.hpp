@@ -23,9 +23,8 @@ struct AxisTables {
   std::vector<float> k;
 };
 
-inline AxisTables build_axis(int n, double d, bool use_sg) {
-  AxisTables t;
-  t.grad.resize(n); t.div.resize(n); t.unst.resize(n); t.tw.resize(n); t.k.resize(n);
+// Writes the n entries of each table into the caller's arrays.
+inline void build_axis_into(int n, double d, bool use_sg, float2* grad, float2* div, float2* unst, float2* tw, float* kv) {
   const double two_pi = 2.0 * M_PI;
   for (int i = 0; i < n; ++i) {
     // numpy.fft.fftfreq ordering: 0..ceil(n/2)-1, then -floor(n/2)..-1
@@ -38,13 +37,19 @@ inline AxisTables build_axis(int n, double d, bool use_sg) {
     if (n % 2 == 0 && i == n / 2) {  // self-conjugate bin: keep the Hermitian part
       g = g.real(); dv = dv.real(); un = un.real();
     }
-    t.grad[i] = make_float2((float)g.real(), (float)g.imag());
-    t.div[i] = make_float2((float)dv.real(), (float)dv.imag());
-    t.unst[i] = make_float2((float)un.real(), (float)un.imag());
-    t.k[i] = (float)k;
+    grad[i] = make_float2((float)g.real(), (float)g.imag());
+    div[i] = make_float2((float)dv.real(), (float)dv.imag());
+    unst[i] = make_float2((float)un.real(), (float)un.imag());
+    kv[i] = (float)k;
     const double a = -two_pi * i / n;
-    t.tw[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+    tw[i] = make_float2((float)std::cos(a), (float)std::sin(a));
   }
+}
+
+inline AxisTables build_axis(int n, double d, bool use_sg) {
+  AxisTables t;
+  t.grad.resize(n); t.div.resize(n); t.unst.resize(n); t.tw.resize(n); t.k.resize(n);
+  build_axis_into(n, d, use_sg, t.grad.data(), t.div.data(), t.unst.data(), t.tw.data(), t.k.data());
   return t;
 }
 

@@ -735,41 +735,60 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COL
 
   float2 v[C::E];
   if constexpr (!OPS) {
-    // software pipeline over the buffers: the loads of buffer q+1 are in flight
-    // while buffer q is transformed
+    // software pipeline over the buffers and tiles: the loads of the next buffer
+    // (after a tile's last buffer, the first buffer of this workgroup's next tile)
+    // are in flight while the current one is transformed and stored
+    const int ntiles = a.nouter * ntile;
     float2 nx[C::E];
     if (valid) {
       const auto bp = gp(fl.spec[a.bufs[0]]);
 #pragma unroll
       for (int m = 0; m < C::E; ++m) nx[m] = ld2(bp, base + (int64_t)(j0 + m * C::TPL) * a.estride);
     }
-    for (int q = 0; q < a.nbuf; ++q) {
-      const auto bp = gw(fl.spec[a.bufs[q]]);
+    for (int lt = lid; lt < ntiles; lt += (int)gridDim.x) {
+      const int ot = lt / ntile, colt = (lt % ntile) * L + c;
+      const bool vt = colt < a.ncols;
+      const int64_t bt = (int64_t)b * g.nspec + (int64_t)ot * a.ostride + colt;
+      const int ltn = lt + (int)gridDim.x;
+      const int coln = (ltn % ntile) * L + c;
+      const bool vn = ltn < ntiles && coln < a.ncols;
+      const int64_t bn = (int64_t)b * g.nspec + (int64_t)(ltn / ntile) * a.ostride + coln;
+      for (int q = 0; q < a.nbuf; ++q) {
+        const auto bp = gw(fl.spec[a.bufs[q]]);
 #pragma unroll
-      for (int m = 0; m < C::E; ++m) v[m] = valid ? nx[m] : make_float2(0.f, 0.f);
-      if (q + 1 < a.nbuf && valid) {
-        const auto bn = gp(fl.spec[a.bufs[q + 1]]);
+        for (int m = 0; m < C::E; ++m) v[m] = vt ? nx[m] : make_float2(0.f, 0.f);
+        if (q + 1 < a.nbuf) {
+          if (vt) {
+            const auto bq = gp(fl.spec[a.bufs[q + 1]]);
 #pragma unroll
-        for (int m = 0; m < C::E; ++m) nx[m] = ld2(bn, base + (int64_t)(j0 + m * C::TPL) * a.estride);
-      }
-      if (a.mode == 0) fft_reg<N, -1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
-      else fft_reg<N, +1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
-      if (valid) {
+            for (int m = 0; m < C::E; ++m) nx[m] = ld2(bq, bt + (int64_t)(j0 + m * C::TPL) * a.estride);
+          }
+        } else if (vn) {
+          const auto bq = gp(fl.spec[a.bufs[0]]);
 #pragma unroll
-        for (int m = 0; m < C::E; ++m) st2(bp, base + (int64_t)(j0 + m * C::TPL) * a.estride, v[m]);
+          for (int m = 0; m < C::E; ++m) nx[m] = ld2(bq, bn + (int64_t)(j0 + m * C::TPL) * a.estride);
+        }
+        if (a.mode == 0) fft_reg<N, -1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
+        else fft_reg<N, +1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
+        if (vt) {
+#pragma unroll
+          for (int m = 0; m < C::E; ++m) st2(bp, bt + (int64_t)(j0 + m * C::TPL) * a.estride, v[m]);
+        }
       }
     }
   } else {
-  // forward, multiply, inverse (outermost axis = axis 0)
+  // forward, multiply, inverse (outermost axis = axis 0), over the tiles lid,
+  // lid + G, ... of this instance (G = workgroups per instance: one tile each when
+  // the grid covers every tile, else a resident grid that loops; with the
+  // XCD-aware lid, the tiles an XCD runs at one time are neighbours).
   const float kap = inst[b].kap;
-  int iy, iz;
-  if (g.nd == 2) { iy = valid ? col : 0; iz = 0; }
-  else { iy = o; iz = valid ? col : 0; }
   // Operator values read between the transforms come from LDS, so no global
   // latency sits between a forward and an inverse transform: the x wavenumbers,
-  // the x-axis operator vectors of up to two tabled jobs, and per column of the
-  // tile the value of every y/z-axis job.  (The twiddle barrier above has passed;
-  // the first transform's barriers publish these before they are read.)
+  // the x-axis operator vectors of up to two tabled jobs (loaded once), and per
+  // column of the tile the value of every y/z-axis job.  (The twiddle barrier
+  // above has passed; the first transform's barriers publish these before they
+  // are read, and the previous tile's last reads of cop sit before its last
+  // transform's barriers.)
   __shared__ float kxs[N];
   __shared__ float2 xop[2][N];
   __shared__ float2 cop[kMaxJobs][L];
@@ -778,99 +797,118 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COL
     const XJob jb = a.job[j];
     const float2* t = jb.kind == XK_GRAD ? ops.grad[jb.axis] : jb.kind == XK_DIV ? ops.div[jb.axis]
                     : jb.kind == XK_UNSTAGGER ? ops.unst[jb.axis] : nullptr;
-    if (!t) continue;
-    if (jb.axis == 0) {
-      if (slot < 2)
-        for (int i_ = tid; i_ < N; i_ += C::NT) xop[slot][i_] = ld2(gp(t), i_);
-      ++slot;
-    } else {
+    if (!t || jb.axis != 0) continue;
+    if (slot < 2)
+      for (int i_ = tid; i_ < N; i_ += C::NT) xop[slot][i_] = ld2(gp(t), i_);
+    ++slot;
+  }
+  const int ntiles = a.nouter * ntile;
+  // The next distinct input is loaded while the current one is transformed and
+  // its jobs stored: issued after those stores, its loads would also wait for
+  // them (the vector-memory counter drains in order).  Within a tile, not when
+  // one of the jobs before it writes that buffer; after a tile's last input, the
+  // next tile's first input (other columns, so never written by this tile).
+  // Only for short register arrays (the mixed-radix ones would lose occupancy).
+  constexpr bool PRE = C::E <= 8;
+  float2 nx[PRE ? C::E : 1];
+  bool next_loaded = false;  // nx holds this tile's first input
+  for (int lt = lid; lt < ntiles; lt += (int)gridDim.x) {
+    const int o = lt / ntile, tile = lt % ntile;
+    const int col = tile * L + c;
+    const bool valid = col < a.ncols;
+    const int64_t base = (int64_t)b * g.nspec + (int64_t)o * a.ostride + col;
+    int iy, iz;
+    if (g.nd == 2) { iy = valid ? col : 0; iz = 0; }
+    else { iy = o; iz = valid ? col : 0; }
+    for (int j = 0; j < a.njob; ++j) {
+      const XJob jb = a.job[j];
+      const float2* t = jb.kind == XK_GRAD ? ops.grad[jb.axis] : jb.kind == XK_DIV ? ops.div[jb.axis]
+                      : jb.kind == XK_UNSTAGGER ? ops.unst[jb.axis] : nullptr;
+      if (!t || jb.axis == 0) continue;
       for (int cc = tid; cc < L; cc += C::NT) {
         const int colc = min(tile * L + cc, a.ncols - 1);
         const int ia = (jb.axis == 1 && g.nd == 3) ? o : colc;
         cop[j][cc] = ld2(gp(t), ia);
       }
     }
-  }
-  float kyz2 = 0.f;
-  {
-    const float ky = gp(ops.k[1])[iy];
-    kyz2 = ky * ky;
-    if (g.nd == 3) { const float kz = gp(ops.k[2])[iz]; kyz2 += kz * kz; }
-  }
-  float2 st[C::E];
-  float kap_m[C::E];  // kappa(|k|) of each owned element, shared by the gradient/divergence jobs
+    float kyz2 = 0.f;
+    {
+      const float ky = gp(ops.k[1])[iy];
+      kyz2 = ky * ky;
+      if (g.nd == 3) { const float kz = gp(ops.k[2])[iz]; kyz2 += kz * kz; }
+    }
+    float2 st[C::E];
+    int cur_in = -1, pre_in = next_loaded ? a.job[0].in : -1;
+    next_loaded = false;
+    for (int j = 0; j < a.njob; ++j) {
+      const XJob jb = a.job[j];
+      if (jb.in != cur_in) {
+        cur_in = jb.in;
+        if (PRE && pre_in == cur_in) {
 #pragma unroll
-  for (int m = 0; m < C::E; ++m) {
-    const float kx = gp(ops.k[0])[j0 + m * C::TPL];
-    kap_m[m] = kap > 0.f ? sinc_k(kap * sqrtf(fmaf(kx, kx, kyz2))) : 1.f;
-  }
-  // The next distinct input is loaded while the current one is transformed and
-  // its jobs stored: issued after those stores, its loads would also wait for
-  // them (the vector-memory counter drains in order).  Not when one of the jobs
-  // before it writes that buffer, and only for short register arrays (the
-  // mixed-radix ones would lose occupancy).
-  constexpr bool PRE = C::E <= 8;
-  float2 nx[PRE ? C::E : 1];
-  int cur_in = -1, pre_in = -1;
-  for (int j = 0; j < a.njob; ++j) {
-    const XJob jb = a.job[j];
-    if (jb.in != cur_in) {
-      cur_in = jb.in;
-      if (PRE && pre_in == cur_in) {
+          for (int m = 0; m < C::E; ++m) st[m] = nx[PRE ? m : 0];
+        } else {
+          const auto src = gp(fl.spec[jb.in]);
 #pragma unroll
-        for (int m = 0; m < C::E; ++m) st[m] = nx[PRE ? m : 0];
-      } else {
-        const auto src = gp(fl.spec[jb.in]);
-#pragma unroll
-        for (int m = 0; m < C::E; ++m) st[m] = valid ? ld2(src, base + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
-      }
-      pre_in = -1;
-      int k = j + 1;
-      while (k < a.njob && a.job[k].in == cur_in) ++k;
-      if (PRE && k < a.njob) {
-        bool clobbered = false;
-        for (int i = j; i < k; ++i) clobbered |= a.job[i].out == a.job[k].in;
-        if (!clobbered) {
-          pre_in = a.job[k].in;
-          const auto nsrc = gp(fl.spec[pre_in]);
-#pragma unroll
-          for (int m = 0; m < C::E; ++m) nx[PRE ? m : 0] = valid ? ld2(nsrc, base + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
+          for (int m = 0; m < C::E; ++m) st[m] = valid ? ld2(src, base + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
         }
-      }
-      fft_reg<N, -1, typename C::Idx, EM>(st, lds, tw, c, j0, idx);
-    }
-    // where this job's operator values live: an x-table slot, the per-column
-    // values, or (a third x-axis table) global memory
-    const bool tabled = jb.kind == XK_GRAD || jb.kind == XK_DIV || jb.kind == XK_UNSTAGGER;
-    int slot = 0;
-    for (int i = 0; i < j; ++i) {
-      const XJob q = a.job[i];
-      slot += q.axis == 0 && (q.kind == XK_GRAD || q.kind == XK_DIV || q.kind == XK_UNSTAGGER);
-    }
-    const float2* gtab = jb.kind == XK_GRAD ? ops.grad[0] : jb.kind == XK_DIV ? ops.div[0] : ops.unst[0];
-    const float2 colv = (tabled && jb.axis != 0) ? cop[j][c] : make_float2(1.f, 0.f);
+        pre_in = -1;
+        int k = j + 1;
+        while (k < a.njob && a.job[k].in == cur_in) ++k;
+        if (PRE && k < a.njob) {
+          bool clobbered = false;
+          for (int i = j; i < k; ++i) clobbered |= a.job[i].out == a.job[k].in;
+          if (!clobbered) {
+            pre_in = a.job[k].in;
+            const auto nsrc = gp(fl.spec[pre_in]);
 #pragma unroll
-    for (int m = 0; m < C::E; ++m) {
-      const int n = j0 + m * C::TPL;
-      const float kx = kxs[n];
-      const float km = sqrtf(fmaf(kx, kx, kyz2));
-      float2 op = make_float2(1.f, 0.f);
-      if (tabled) op = jb.axis != 0 ? colv : (slot < 2 ? xop[slot][n] : ld2(gp(gtab), n));
-      switch (jb.kind) {
-        case XK_GRAD:
-        case XK_DIV: op = cscale(op, kap_m[m]); break;
-        case XK_FRAC1: op = make_float2(kpow(km, ops.frac1), 0.f); break;
-        case XK_FRAC2: op = make_float2(kpow(km, ops.frac2), 0.f); break;
-        case XK_SRCKAPPA: op = make_float2(kap > 0.f ? cos_k(kap * km) : 1.f, 0.f); break;
-        default: break;
-      }
-      v[m] = cscale(cmul(st[m], op), ops.inv_ntot);
-    }
-    fft_reg<N, +1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
-    if (valid) {
-      const auto dst = gw(fl.spec[jb.out]);
+            for (int m = 0; m < C::E; ++m) nx[PRE ? m : 0] = valid ? ld2(nsrc, base + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
+          }
+        } else if (PRE && lt + (int)gridDim.x < ntiles) {
+          const int ltn = lt + (int)gridDim.x;
+          const int on = ltn / ntile, coln = (ltn % ntile) * L + c;
+          const bool validn = coln < a.ncols;
+          const int64_t basen = (int64_t)b * g.nspec + (int64_t)on * a.ostride + coln;
+          const auto nsrc = gp(fl.spec[a.job[0].in]);
 #pragma unroll
-      for (int m = 0; m < C::E; ++m) st2(dst, base + (int64_t)(j0 + m * C::TPL) * a.estride, v[m]);
+          for (int m = 0; m < C::E; ++m) nx[PRE ? m : 0] = validn ? ld2(nsrc, basen + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
+          next_loaded = true;
+        }
+        fft_reg<N, -1, typename C::Idx, EM>(st, lds, tw, c, j0, idx);
+      }
+      // where this job's operator values live: an x-table slot, the per-column
+      // values, or (a third x-axis table) global memory
+      const bool tabled = jb.kind == XK_GRAD || jb.kind == XK_DIV || jb.kind == XK_UNSTAGGER;
+      int slot = 0;
+      for (int i = 0; i < j; ++i) {
+        const XJob q = a.job[i];
+        slot += q.axis == 0 && (q.kind == XK_GRAD || q.kind == XK_DIV || q.kind == XK_UNSTAGGER);
+      }
+      const float2* gtab = jb.kind == XK_GRAD ? ops.grad[0] : jb.kind == XK_DIV ? ops.div[0] : ops.unst[0];
+      const float2 colv = (tabled && jb.axis != 0) ? cop[j][c] : make_float2(1.f, 0.f);
+#pragma unroll
+      for (int m = 0; m < C::E; ++m) {
+        const int n = j0 + m * C::TPL;
+        const float kx = kxs[n];
+        const float km = sqrtf(fmaf(kx, kx, kyz2));
+        float2 op = make_float2(1.f, 0.f);
+        if (tabled) op = jb.axis != 0 ? colv : (slot < 2 ? xop[slot][n] : ld2(gp(gtab), n));
+        switch (jb.kind) {
+          case XK_GRAD:
+          case XK_DIV: op = cscale(op, kap > 0.f ? sinc_k(kap * km) : 1.f); break;  // kappa(|k|)
+          case XK_FRAC1: op = make_float2(kpow(km, ops.frac1), 0.f); break;
+          case XK_FRAC2: op = make_float2(kpow(km, ops.frac2), 0.f); break;
+          case XK_SRCKAPPA: op = make_float2(kap > 0.f ? cos_k(kap * km) : 1.f, 0.f); break;
+          default: break;
+        }
+        v[m] = cscale(cmul(st[m], op), ops.inv_ntot);
+      }
+      fft_reg<N, +1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
+      if (valid) {
+        const auto dst = gw(fl.spec[jb.out]);
+#pragma unroll
+        for (int m = 0; m < C::E; ++m) st2(dst, base + (int64_t)(j0 + m * C::TPL) * a.estride, v[m]);
+      }
     }
   }
   }

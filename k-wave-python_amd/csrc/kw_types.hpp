@@ -102,6 +102,8 @@ struct ColArgs {
   int ncols;        // columns (= nh)
   int tile;         // 0: default tile width, 1: the alternative (chosen per plan by timing)
   int wide;         // 1: keep the chosen width on small grids too (parity tests of the large-grid tiles)
+  int persist;      // passes whose workgroups may loop over tiles: bit 0 with jobs, bit 1 FFT-only
+  int grid;         // > 0: workgroups per instance when looping (tests of the loop on small grids)
 };
 
 struct Fields {

@@ -114,6 +114,7 @@ def problem_from_file(v):
 
 def run_input_file(input_file, output_file, *, device=0, record=("p",)):
     """Execute an input file on the GPU and write the output file; returns the output variables written."""
+    from .. import _abi
     from .kspace_solver import _resolve_dtype, collect_results
     from .plan import DevicePlan
     from .setup import build_instance
@@ -142,6 +143,12 @@ def run_input_file(input_file, output_file, *, device=0, record=("p",)):
     try:
         plan.run(st.Nt - plan.step_index(0))
         res = collect_results(plan, 0, st, _resolve_dtype(np.float32))
+        # the binaries write final fields over the whole grid, PML included (the caller crops them)
+        if "p_final" in want:
+            res["p_final"] = plan.field(0, _abi.F_P).copy()
+        if "u_final" in want:
+            for i, a in enumerate("xyz"[:nd]):
+                res[f"u{a}_final"] = plan.field(0, _abi.F_UX + i).copy()
     finally:
         plan.close()
     # rows in Fortran order of the sensor points, as the binaries write them

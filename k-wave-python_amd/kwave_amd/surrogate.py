@@ -6,22 +6,30 @@ Training data come from the GPU solver (batched plans); the solver also measures
 the surrogate's error and is its fallback.
 
 Network (runs at 64^2, the c map averaged over 2 x 2 cells):
-  lift     (c, x, y) -> 16 channels, pointwise
-  3 x      v <- GELU(SpectralConv(v) + W v + b)   (16 x 16 retained modes, both kx signs)
+  lift     (c, x, y) -> 32 channels, pointwise
+  4 x      v <- GELU(SpectralConv(v) + W v + b)   (16 x 16 retained modes, both kx signs)
   head     for every sensor, the feature columns at its 64^2 column and the
-           neighbour on its side (2 x 16 channels x 64 depths) and the full-resolution
+           neighbour on its side (2 x 32 channels x 64 depths) and the full-resolution
            c column under the sensor (128 depths; the injected amplitude scales with
            the local c) -> 600 samples by one shared linear map, plus a learned
            per-sensor mean trace.
 
-The spectral contraction of every layer - forward, input gradient and weight
-gradient - runs in the library's BF16 MFMA kernels (kw_mode_contract_bf16,
-kw_mode_gemm_bf16, FP32 accumulation).  Only 2 md x md of the 64 x 33 Fourier
-modes are kept, so the truncated transforms are dense matrix products (two per
-direction, one per axis) rather than full FFTs; pointwise layers and the
-optimiser are PyTorch; at inference both truncated transforms of a layer run in
-one MFMA kernel (kw_fno_dft_bf16) and its pointwise part (channel mix, bias,
-residual sum, GELU) in another (kw_fno_pointwise_bf16).
+The truncated 2-D transforms of every layer run in the library's BF16 MFMA kernel
+(kw_fno_dft_bf16), in training as well: each transform is linear, so its gradient
+is the opposite-direction kernel with transposed matrices.  The spectral
+contraction - forward, input gradient and weight gradient - runs in
+kw_mode_contract_bf16 / kw_mode_gemm_bf16 (FP32 accumulation).  Only 2 md x md of
+the 64 x 33 Fourier modes are kept, so the truncated transforms are dense matrix
+products rather than full FFTs.  At inference the pointwise part of a layer
+(channel mix, bias, residual sum, GELU) is one kernel (kw_fno_pointwise_bf16);
+in training it and the optimiser are PyTorch.
+
+Per-map fallback: an error head (a small MLP on pooled last-layer features and a
+few scalars of the map and of the prediction) estimates each map's relative L2
+error; it is fitted on solver-checked calibration maps, and the routing threshold
+is chosen on a second calibration split so that few accepted maps exceed the
+error bound.  Maps above the threshold, and maps outside the training range, go
+to the solver.
 """
 from __future__ import annotations
 
@@ -35,8 +43,8 @@ from . import _abi, workloads
 N_FINE = 128
 N_COARSE = 64
 MODES = 16
-WIDTH = 16
-LAYERS = 3
+WIDTH = 32
+LAYERS = 4
 TRACE_SCALE = 1e5  # Pa; traces are learned in units of the source amplitude
 C_MID, C_HALF = 1525.0, 125.0
 
@@ -154,6 +162,54 @@ def fno_dft(direction, x, m1, m2):
                                           C.c_void_p(m1.data_ptr()), C.c_void_p(m2.data_ptr()), C.c_void_p(out.data_ptr()),
                                           C.c_void_p(stream)))
     return out
+
+
+def _dft_fns():
+    """Truncated 2-D DFT of 64 x 64 slabs (16 modes) on the project's kernels, with gradients.
+
+    Both transforms are linear, X = view(Mx v) My and v' = Nx view(Y Ny) (dft_mats), so each
+    backward is the opposite-direction kernel with transposed matrices:
+        dL/dv = Mx^T view(G My^T)   -> kw_fno_dft_bf16(-1) with (My, Mx^T) in the (Ny^T, Nx) slots
+        dL/dY = view(Nx^T G) Ny^T   -> kw_fno_dft_bf16(+1) with (Nx^T, Ny) in the (Mx, My^T) slots
+    Activations on the v side are bf16, spectra fp32 (FP32 accumulation throughout).
+    """
+    import torch
+
+    class DftFwd(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, v, conv):
+            ctx.conv = conv
+            return fno_dft(1, v.to(torch.bfloat16), conv.mx_b, conv.myt_b)
+
+        @staticmethod
+        def backward(ctx, g):
+            c = ctx.conv
+            return fno_dft(-1, g.float().contiguous(), c.my_b, c.mxt_b), None
+
+    class DftInv(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, y, conv):
+            ctx.conv = conv
+            return fno_dft(-1, y.float().contiguous(), conv.nyt_b, conv.nx_b)
+
+        @staticmethod
+        def backward(ctx, g):
+            c = ctx.conv
+            return fno_dft(1, g.to(torch.bfloat16).contiguous(), c.nxt_b, c.ny_b), None
+
+    return DftFwd, DftInv
+
+
+_DFT = None
+
+
+def fno_dft_train(direction, x, conv):
+    """fno_dft with gradients (training): direction 1 maps bf16 (..., 64, 64) slabs to fp32 (..., 32, 32)
+    retained modes, -1 maps them back; the matrices come from the SpectralConv2d module ``conv``."""
+    global _DFT
+    if _DFT is None:
+        _DFT = _dft_fns()
+    return (_DFT[0] if direction == 1 else _DFT[1]).apply(x, conv)
 
 
 def head_gather(v, cols):
@@ -302,10 +358,12 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
             mats = dft_mats(n, modes)
             for name, a in zip(("mx", "my", "ny", "nx"), mats):
                 self.register_buffer(name, torch.as_tensor(a))
-            # bf16 operands of the fused inference transforms (kw_fno_dft_bf16: 64 x 64 slabs, 16 modes)
+            # bf16 operands of the fused transforms (kw_fno_dft_bf16: 64 x 64 slabs, 16 modes), and the
+            # transposed pairs their gradients use
             self.fused_dft = n == 64 and modes == 16
             mx, my, ny_, nx_ = mats
-            for name, a in (("mx_b", mx), ("myt_b", my.T), ("nyt_b", ny_.T), ("nx_b", nx_)):
+            for name, a in (("mx_b", mx), ("myt_b", my.T), ("nyt_b", ny_.T), ("nx_b", nx_),
+                            ("my_b", my), ("mxt_b", mx.T), ("nxt_b", nx_.T), ("ny_b", ny_)):
                 self.register_buffer(name, torch.as_tensor(np.ascontiguousarray(a)).to(torch.bfloat16), persistent=False)
 
         def weights(self):
@@ -316,10 +374,11 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
         def forward(self, v):
             b, c, n, _ = v.shape
             md = self.modes
-            if self.fused_dft and v.dtype == torch.bfloat16 and v.is_cuda and not torch.is_grad_enabled():
-                xm = fno_dft(1, v, self.mx_b, self.myt_b).reshape(b, c, 2 * md, md, 2)
-                ym = spectral_modes(torch.view_as_complex(xm), self.weights(), md)
-                return fno_dft(-1, torch.view_as_real(ym).reshape(b, c, 2 * md, 2 * md), self.nyt_b, self.nx_b)
+            if self.fused_dft and v.is_cuda:
+                # both truncated transforms on the project's kernels, in inference and in training
+                xm = fno_dft_train(1, v, self).reshape(b, c, 2 * md, md, 2)
+                ym = spectral_modes(torch.view_as_complex(xm.contiguous()), self.weights(), md)
+                return fno_dft_train(-1, torch.view_as_real(ym).reshape(b, c, 2 * md, 2 * md), self)
             bx = torch.matmul(self.mx, v).reshape(b, c, 2 * md, 2 * n)
             xm = torch.matmul(bx, self.my).float().reshape(b, c, 2 * md, md, 2)  # retained modes
             ym = spectral_modes(torch.view_as_complex(xm), self.weights(), md)
@@ -353,7 +412,11 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
             self.register_buffer("grid", torch.stack(torch.meshgrid(g, g, indexing="ij"))[None])
             self.n_sens = n_sens
 
-        def forward(self, cmap):
+        def forward(self, cmap, features=False):
+            """Traces (b, n_sens, nt) in units of TRACE_SCALE; with ``features`` also the
+            error head's inputs (b, 2 width + 3): per-channel mean and standard deviation of
+            the last layer over the grid, the standard deviation and largest magnitude of the
+            normalised map, and the prediction's distance from the mean trace set."""
             b = cmap.shape[0]
             n = N_COARSE
             cn = (cmap - C_MID) / C_HALF
@@ -380,7 +443,15 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
             wab = torch.cat([self.head_a.weight, self.head_b.weight], 1)
             ha = F.linear(sel, wab, self.head_a.bias).reshape(b, self.n_sens, -1)
             ff = cn.index_select(2, self.cols_f).permute(0, 2, 1)                # (b, n_sens, 128)
-            return self.mean[None] + ha + self.head_f(ff)
+            out = self.mean[None] + ha + self.head_f(ff)
+            if not features:
+                return out
+            vf = v.float()
+            dev = (out.float() - self.mean[None]).flatten(1).norm(dim=1) / self.mean.norm()
+            flat = cn.flatten(1)
+            feats = torch.cat([vf.mean((2, 3)), vf.std((2, 3)), flat.std(1, keepdim=True),
+                               flat.abs().amax(1, keepdim=True), dev[:, None]], 1)
+            return out, feats
 
     kg, med, src, sen, pml, alpha = workloads.surrogate_2d(0)
     ys = np.where(np.asarray(sen.mask).any(axis=0))[0]
@@ -460,29 +531,108 @@ def train(model, cmaps, traces, epochs=40, batch=64, lr=2e-3, log=None):
     return hist
 
 
-def predict(model, cmaps, batch=512):
-    """Surrogate traces (n, 32, 600) in Pa (pointwise layers and head in bf16 autocast)."""
+def predict(model, cmaps, batch=512, features=False):
+    """Surrogate traces (n, 32, 600) in Pa (pointwise layers and head in bf16 autocast); with
+    ``features`` also the error head's inputs (n, 2 width + 3) as float32."""
     import torch
 
     dev = next(model.parameters()).device
-    out = []
+    out, fts = [], []
     with torch.no_grad(), torch.autocast("cuda", dtype=torch.bfloat16):
         for i in range(0, len(cmaps), batch):
-            out.append(model(torch.as_tensor(cmaps[i:i + batch], device=dev)).float())
-    return (torch.cat(out) * TRACE_SCALE).cpu().numpy()
+            r = model(torch.as_tensor(cmaps[i:i + batch], device=dev), features=features)
+            if features:
+                r, f = r
+                fts.append(f.float())
+            out.append(r.float())
+    traces = (torch.cat(out) * TRACE_SCALE).cpu().numpy()
+    return (traces, torch.cat(fts).cpu().numpy()) if features else traces
+
+
+def build_error_head(n_in, device="cuda", seed=1, hidden=64):
+    """MLP estimating log10 of a map's relative L2 error from the forward's features."""
+    import torch
+    import torch.nn as nn
+
+    torch.manual_seed(seed)
+
+    class ErrorHead(nn.Module):
+        def __init__(self):
+            super().__init__()
+            self.register_buffer("mu", torch.zeros(n_in))
+            self.register_buffer("sd", torch.ones(n_in))
+            self.net = nn.Sequential(nn.Linear(n_in, hidden), nn.GELU(), nn.Linear(hidden, hidden), nn.GELU(),
+                                     nn.Linear(hidden, 1))
+
+        def forward(self, f):
+            return self.net((f - self.mu) / self.sd)[:, 0]
+
+    return ErrorHead().to(device)
+
+
+def fit_error_head(feats, err, steps=3000, lr=3e-3, weight_decay=1e-4, log=None):
+    """Fit the error head to log10(err) of solver-checked maps (full-batch Adam, cosine decay)."""
+    import torch
+
+    head = build_error_head(feats.shape[1])
+    dev = head.mu.device
+    f = torch.as_tensor(feats, dtype=torch.float32, device=dev)
+    t = torch.as_tensor(np.log10(np.maximum(err, 1e-6)), dtype=torch.float32, device=dev)
+    with torch.no_grad():
+        head.mu.copy_(f.mean(0))
+        head.sd.copy_(f.std(0).clamp_min(1e-6))
+    opt = torch.optim.AdamW(head.parameters(), lr=lr, weight_decay=weight_decay)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, steps)
+    for _ in range(steps):
+        loss = ((head(f) - t) ** 2).mean()
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        sched.step()
+    if log:
+        log(f"error head: fit rms of log10 error {float(loss.detach()) ** 0.5:.3f}")
+    return head.eval()
+
+
+def predicted_error(head, feats):
+    """The error head's estimate of each map's relative L2 error."""
+    import torch
+
+    with torch.no_grad():
+        z = head(torch.as_tensor(feats, dtype=torch.float32, device=head.mu.device))
+    return np.power(10.0, z.double().cpu().numpy())
+
+
+def routing_threshold(pred_err, err, max_err, miss_rate):
+    """Largest threshold on the predicted error such that, among these maps, the ones
+    accepted (predicted <= threshold) with a true error above ``max_err`` are at most
+    ``miss_rate`` of all maps."""
+    order = np.argsort(pred_err, kind="stable")
+    bad = np.cumsum(err[order] > max_err)
+    ok = np.nonzero(bad <= miss_rate * len(err))[0]
+    if len(ok) == 0:
+        return 0.0
+    k = ok[-1]
+    # accept everything up to and including order[k]; stop below the next predicted value
+    return float(pred_err[order[k]]) if k + 1 == len(order) else float(
+        0.5 * (pred_err[order[k]] + pred_err[order[k + 1]]))
 
 
 class Surrogate:
-    """Trained network with the solver as fallback.
+    """Trained network with the solver as per-map fallback.
 
-    ``__call__`` returns traces for c maps; maps outside the training range
-    [1400, 1650] m/s go to the solver.  ``validate`` measures the error on
-    solver-checked maps; if the measured mean error exceeds ``max_err`` (2 %),
-    the surrogate disables itself and every later call goes to the solver.
+    ``__call__`` returns traces for c maps and the mask of maps the solver ran.
+    A map goes to the solver when it lies outside the training range
+    [1400, 1650] m/s, or, once ``calibrate`` has fitted the error head, when the
+    head's estimate of its error exceeds the calibrated threshold.  ``validate``
+    measures the error on solver-checked maps; if the measured mean error exceeds
+    ``max_err`` (2 %), the surrogate disables itself and every later call goes to
+    the solver.
     """
 
     def __init__(self, model, max_err=0.02):
         self.model, self.max_err, self.enabled = model, max_err, True
+        self.head, self.threshold = None, None
 
     def validate(self, cmaps, traces=None):
         traces = solve_batch(np.asarray(cmaps, dtype=np.float64)) if traces is None else traces
@@ -490,32 +640,71 @@ class Surrogate:
         self.enabled = bool(err.mean() <= self.max_err)
         return err
 
-    def __call__(self, cmaps):
+    def calibrate(self, cmaps, traces=None, miss_rate=0.005, fit_fraction=0.6, log=None):
+        """Fit the error head on the first ``fit_fraction`` of these solver-checked maps and
+        choose the routing threshold on the rest.  Returns the calibration summary."""
+        cmaps = np.asarray(cmaps, dtype=np.float32)
+        traces = solve_batch(cmaps.astype(np.float64)) if traces is None else traces
+        pred, feats = predict(self.model, cmaps, features=True)
+        err = rel_l2(pred, traces)
+        nf = int(round(fit_fraction * len(cmaps)))
+        if nf < 16 or len(cmaps) - nf < 16:
+            raise ValueError("calibrate needs at least 16 maps in each split")
+        self.head = fit_error_head(feats[:nf], err[:nf], log=log)
+        pe = predicted_error(self.head, feats[nf:])
+        self.threshold = routing_threshold(pe, err[nf:], self.max_err, miss_rate)
+        routed = pe > self.threshold
+        return {"n_fit": nf, "n_threshold": len(cmaps) - nf, "threshold": self.threshold,
+                "routed_frac": float(routed.mean()),
+                "miss_frac": float(((err[nf:] > self.max_err) & ~routed).mean()),
+                "log10_err_corr": float(np.corrcoef(np.log10(pe), np.log10(np.maximum(err[nf:], 1e-6)))[0, 1])}
+
+    def route(self, cmaps):
+        """(traces of the surrogate for every map, mask of maps to send to the solver)."""
         cmaps = np.asarray(cmaps, dtype=np.float32)
         lo, hi = workloads.SURROGATE_C_RANGE
-        ood = (cmaps.reshape(len(cmaps), -1).min(1) < lo - 1e-3) | (cmaps.reshape(len(cmaps), -1).max(1) > hi + 1e-3)
+        flat = cmaps.reshape(len(cmaps), -1)
+        to_solver = (flat.min(1) < lo - 1e-3) | (flat.max(1) > hi + 1e-3)
         if not self.enabled:
-            ood[:] = True
+            to_solver[:] = True
+            return None, to_solver
+        if self.head is None:
+            return predict(self.model, cmaps), to_solver
+        pred, feats = predict(self.model, cmaps, features=True)
+        to_solver |= predicted_error(self.head, feats) > self.threshold
+        return pred, to_solver
+
+    def __call__(self, cmaps):
+        cmaps = np.asarray(cmaps, dtype=np.float32)
+        pred, to_solver = self.route(cmaps)
         out = np.empty((len(cmaps),) + tuple(self.model.mean.shape), dtype=np.float32)
-        if (~ood).any():
-            out[~ood] = predict(self.model, cmaps[~ood])
-        if ood.any():
-            out[ood] = solve_batch(cmaps[ood].astype(np.float64))
-        return out, ood
+        if pred is not None:
+            out[~to_solver] = pred[~to_solver]
+        if to_solver.any():
+            out[to_solver] = solve_batch(cmaps[to_solver].astype(np.float64))
+        return out, to_solver
 
 
-def time_surrogate(model, cmaps, batch=1024, reps=5):
-    """Device time per sample of the surrogate forward (inputs already on the device)."""
+def time_surrogate(model, cmaps, batch=1024, reps=5, head=None):
+    """Device time per sample of the surrogate forward (inputs already on the device); with an
+    error ``head``, the forward with the head's features and the head itself."""
     import torch
 
     x = torch.as_tensor(cmaps, device=next(model.parameters()).device)
     xb = x.repeat((batch + len(x) - 1) // len(x), 1, 1)[:batch].contiguous()
+
+    def once():
+        if head is None:
+            return model(xb)
+        _, f = model(xb, features=True)
+        return head(f.float())
+
     with torch.no_grad(), torch.autocast("cuda", dtype=torch.bfloat16):
-        model(xb)
+        once()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
-            model(xb)
+            once()
         torch.cuda.synchronize()
     return (time.perf_counter() - t0) / (reps * batch)
 
@@ -553,13 +742,19 @@ def _torch_first():
     torch.cuda.init()
 
 
-def benchmark(n_train=4096, n_test=512, epochs=80, log=print):
-    """Train on solver data, then report held-out error and surrogate vs batched-solver throughput."""
+def benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=24, max_err=0.02, miss_rate=0.005, log=print):
+    """Train on solver data, calibrate the per-map fallback, then report held-out error before
+    and after routing and the surrogate's speed against the batched solver, with the fallback's
+    solver time included.
+
+    Seeds: training 5,000,000 + j, held-out 6,000,000 + j, calibration 7,000,000 + j.
+    """
     _torch_first()
     t0 = time.perf_counter()
     xtr, ytr = make_dataset(5_000_000 + np.arange(n_train))
     t_gen = time.perf_counter() - t0
     xte, yte = make_dataset(6_000_000 + np.arange(n_test))
+    xca, yca = make_dataset(7_000_000 + np.arange(n_cal))
     log(f"surrogate: {n_train} training samples generated in {t_gen:.1f} s")
     model = build_model()
     t0 = time.perf_counter()
@@ -568,41 +763,69 @@ def benchmark(n_train=4096, n_test=512, epochs=80, log=print):
     pred = predict(model, xte)
     err = rel_l2(pred, yte)
     base = rel_l2(np.broadcast_to(ytr.mean(0), yte.shape), yte)
-    # throughput, inputs resident on the device: surrogate forward vs the batched solver's time loop
+    sur = Surrogate(model, max_err=max_err)
+    cal = sur.calibrate(xca, yca, miss_rate=miss_rate, log=log)
+    _, routed = sur.route(xte)
+    after = np.where(routed, 0.0, err)  # routed maps carry the solver's own traces
+    # throughput, inputs resident on the device: surrogate forward (+ error head) vs the batched
+    # solver's time loop; with the fallback, the routed fraction of maps also pays the solver
     t_sur = time_surrogate(model, xte)
+    t_sur_h = time_surrogate(model, xte, head=sur.head)
     t_sol = time_solver(xte.astype(np.float64))
-    # host to host: c maps in host memory -> traces in host memory (solver incl. its host set-up)
+    t_eff = t_sur_h + routed.mean() * t_sol
+    # host to host: c maps in host memory -> traces in host memory (the solver incl. its host set-up)
     t0 = time.perf_counter()
     predict(model, xte)
-    t_sur_h = (time.perf_counter() - t0) / n_test
+    t_sur_hh = (time.perf_counter() - t0) / n_test
+    t0 = time.perf_counter()
+    out, used = sur(xte)
+    t_fb_hh = (time.perf_counter() - t0) / n_test
     t0 = time.perf_counter()
     solve_batch(xte.astype(np.float64), plan_batch=n_test)
-    t_sol_h = (time.perf_counter() - t0) / n_test
-    res = {"n_train": n_train, "n_test": n_test, "epochs": epochs, "train_s": t_train, "datagen_s": t_gen,
+    t_sol_hh = (time.perf_counter() - t0) / n_test
+    err_out = rel_l2(out, yte)
+    res = {"n_train": n_train, "n_test": n_test, "n_cal": n_cal, "epochs": epochs, "train_s": t_train,
+           "datagen_s": t_gen, "width": model.width, "layers": model.layers, "modes": MODES, "grid": N_COARSE,
            "test_rel_l2_mean": float(err.mean()), "test_rel_l2_p95": float(np.percentile(err, 95)),
-           "test_frac_above_2pct": float((err > 0.02).mean()), "mean_trace_baseline_rel_l2": float(base.mean()),
-           "surrogate_ms_per_sample": 1e3 * t_sur, "solver_ms_per_sample_batched": 1e3 * t_sol,
-           "speedup_vs_batched_solver": t_sol / t_sur,
-           "host_to_host_surrogate_ms_per_sample": 1e3 * t_sur_h, "host_to_host_solver_ms_per_sample": 1e3 * t_sol_h,
-           "host_to_host_speedup": t_sol_h / t_sur_h}
+           "test_frac_above_2pct": float((err > max_err).mean()), "mean_trace_baseline_rel_l2": float(base.mean()),
+           "calibration": cal, "routed_frac": float(routed.mean()),
+           "after_routing_rel_l2_mean": float(after.mean()), "after_routing_rel_l2_p95": float(np.percentile(after, 95)),
+           "after_routing_frac_above_2pct": float((after > max_err).mean()),
+           "after_routing_output_rel_l2_max": float(err_out.max()),
+           "surrogate_ms_per_sample": 1e3 * t_sur, "surrogate_with_error_head_ms_per_sample": 1e3 * t_sur_h,
+           "solver_ms_per_sample_batched": 1e3 * t_sol, "speedup_vs_batched_solver": t_sol / t_sur,
+           "speedup_with_fallback": t_sol / t_eff,
+           "host_to_host_surrogate_ms_per_sample": 1e3 * t_sur_hh,
+           "host_to_host_with_fallback_ms_per_sample": 1e3 * t_fb_hh,
+           "host_to_host_solver_ms_per_sample": 1e3 * t_sol_hh,
+           "host_to_host_speedup": t_sol_hh / t_sur_hh, "host_to_host_speedup_with_fallback": t_sol_hh / t_fb_hh}
     log(f"surrogate: held-out rel-L2 {err.mean():.4f} (mean-trace baseline {base.mean():.4f}), "
-        f"{1e3 * t_sur:.4f} ms/sample vs solver {1e3 * t_sol:.3f} ms/sample -> {t_sol / t_sur:.0f}x")
+        f"{100 * (err > max_err).mean():.1f} % above {100 * max_err:.0f} %; routed {100 * routed.mean():.1f} % -> "
+        f"{100 * (after > max_err).mean():.2f} % above; {1e3 * t_sur:.4f} ms/sample vs solver {1e3 * t_sol:.3f} -> "
+        f"{t_sol / t_sur:.0f}x, {t_sol / t_eff:.1f}x with the fallback")
     return res, model
 
 
 def _main():
     """`python -m kwave_amd.surrogate --forward [--reps R]`: time the inference forward
     of an untrained network on 1024 maps (the kernels are the trained network's; used
-    for profiling), or with no flag run the full benchmark."""
+    for profiling), or with no flag run the full benchmark (training, calibration and
+    held-out set sizes and epochs selectable) and print its results as one JSON line."""
     import argparse
+    import json
 
     ap = argparse.ArgumentParser(description=_main.__doc__)
     ap.add_argument("--forward", action="store_true", help="time the forward only (no data, no training)")
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--train", type=int, default=20000)
+    ap.add_argument("--test", type=int, default=1000)
+    ap.add_argument("--cal", type=int, default=3000)
+    ap.add_argument("--epochs", type=int, default=24)
     args = ap.parse_args()
     _torch_first()
     if not args.forward:
-        benchmark()
+        res, _ = benchmark(n_train=args.train, n_test=args.test, n_cal=args.cal, epochs=args.epochs)
+        print(json.dumps(res), flush=True)
         return
     model = build_model().eval()
     cmaps = np.stack([workloads.surrogate_c_map(6_000_000 + i) for i in range(64)]).astype(np.float32)

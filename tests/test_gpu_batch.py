@@ -72,3 +72,18 @@ def test_batch_with_different_alpha_power_equals_looped():
         one = Simulation(kg, med, copy.deepcopy(src), sen, pml_size=pml, pml_alpha=alpha).run()
         assert np.array_equal(got["p"], one["p"])
     assert not np.allclose(batched[0]["p"], batched[1]["p"])
+
+
+def test_batch_graph_equals_per_plan_graphs():
+    """kw_batch_run (one graph per block of steps, size groups as parallel branches) and the
+    plans replaying their own graphs give the same bits, with a second run on fresh plans."""
+    from kwave_amd.batch import BatchSimulation
+
+    cases = _cases()
+    outs = []
+    for graphs in ("batch", "plans", "batch"):
+        bs = BatchSimulation(cases)
+        outs.append(bs.run(graphs=graphs))
+        bs.close()
+    for a, b, c in zip(*outs):
+        assert np.array_equal(a["p"], b["p"]) and np.array_equal(a["p"], c["p"])

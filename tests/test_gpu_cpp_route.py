@@ -40,7 +40,9 @@ def _case(kind):
     elif kind == "velocity":
         m[12, 20:30] = True
         src.u_mask = m
-        src.ux = 1e-2 * base[None, :]
+        # one series per source point for both components (the reference's validation refuses
+        # a mix of single and per-point series)
+        src.ux = np.stack([1e-2 * np.roll(base, 2 * k) for k in range(int(m.sum()))])
         src.uy = np.stack([-5e-3 * np.roll(base, k) for k in range(int(m.sum()))])
     else:
         m[12, 20:36] = True

@@ -69,6 +69,29 @@ def test_column_tile_variants_bit_identical(shape, monkeypatch):
         assert np.array_equal(outs[0][k], outs[1][k]), (shape, k)
 
 
+@pytest.mark.parametrize("shape", [(64, 80), (24, 32, 40), (40, 512, 24), (512, 24, 40)])
+@pytest.mark.parametrize("grid", ["8", "16", "24"])
+def test_looping_column_passes_bit_identical(shape, grid, monkeypatch):
+    """Column passes whose workgroups loop over tiles (and load the next tile's
+    input during the current tile) give the bits of one workgroup per tile.  The
+    benchmark grids loop only at full size; KW_COL_GRID forces a few workgroups
+    per instance here, so every workgroup runs several tiles, with ragged last
+    tiles and the 512-point kernels of the benchmark (KW_COL_WIDE keeps their
+    large-grid tile widths)."""
+    from kwave_amd.solvers.kspace_solver import Simulation
+
+    kg, med, src, sen = _case(shape, nt=40)
+    pml = tuple(4 for _ in shape)
+    outs = []
+    monkeypatch.setenv("KW_COL_WIDE", "1")
+    for persist, g in (("0", "0"), ("3", grid)):
+        monkeypatch.setenv("KW_COL_PERSIST", persist)
+        monkeypatch.setenv("KW_COL_GRID", g)
+        outs.append(Simulation(kg, med, copy.deepcopy(src), sen, pml_size=pml, pml_alpha=tuple(2.0 for _ in shape)).run())
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), (shape, grid, k)
+
+
 def _source_case(shape, per_point, nt=83, slen=29):
     """Multi-row additive p and u sources whose signals end before nt: one
     signal row (the dense-pattern form) or one row per point (the spectral

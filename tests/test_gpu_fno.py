@@ -4,10 +4,14 @@
   MFMA lane layout the kernel assumes);
 * forward, input gradient and weight gradients of the spectral contraction match
   the FP64 CPU reference (oracle/fno_ref.py) on the same bf16-rounded operands;
+* the truncated DFTs of training (kw_fno_dft_bf16 with its gradients) against FP64
+  products of the same matrices, and the network's parameter gradients against the
+  FP32 matrix-product form of the transforms;
 * a short training run on solver data beats the method without the network
   (predicting the training-mean traces) on held-out maps;
-* the surrogate hands out-of-range maps to the solver and disables itself when
-  validation shows more than the allowed error.
+* the surrogate hands out-of-range maps to the solver, routes maps whose estimated
+  error is above the calibrated threshold to it (those carry the solver's traces),
+  and disables itself when validation shows more than the allowed error.
 """
 import numpy as np
 import pytest
@@ -74,10 +78,18 @@ def test_surrogate_training_beats_baseline_and_falls_back():
     print(f"held-out rel-L2: surrogate {err.mean():.4f}, mean-trace baseline {base.mean():.4f}")
     assert err.mean() < 0.7 * base.mean(), (err.mean(), base.mean())
 
+    xca, yca = surrogate.make_dataset(7_000_000 + np.arange(256))
     sur = surrogate.Surrogate(model)
+    cal = sur.calibrate(xca, yca, miss_rate=0.01)
+    print("calibration:", cal)
+    assert cal["miss_frac"] <= 0.01 + 1e-12 and sur.threshold is not None
+    out, used_solver = sur(xte)
+    err = surrogate.rel_l2(out, yte)
+    assert np.all(err[used_solver] == 0.0)            # routed maps carry the solver's traces
+    print(f"routed {used_solver.mean():.3f}; after routing {np.mean(err > 0.02):.3f} of maps above 2 %")
     odd = xte[:2].copy()
     odd[1, 40:50, 60:70] = workloads.SURROGATE_C_RANGE[1] + 50.0  # outside the training range
-    out, used_solver = sur(odd)
+    out, used_solver = surrogate.Surrogate(model)(odd)                # no error head: range check only
     assert used_solver.tolist() == [False, True]
     assert np.array_equal(out[1], surrogate.solve_batch(odd[1:].astype(np.float64))[0])
     strict = surrogate.Surrogate(model, max_err=1e-9)
@@ -85,6 +97,58 @@ def test_surrogate_training_beats_baseline_and_falls_back():
     assert not strict.enabled
     out, used_solver = strict(xte[:2])
     assert used_solver.all()
+
+
+def test_training_dft_gradients_match_fp64():
+    """The kernel-backed truncated DFTs of training: forward values and the gradients (the
+    opposite-direction kernel with transposed matrices) against FP64 products."""
+    import torch
+
+    from kwave_amd import surrogate
+
+    conv = surrogate.build_model().spec[0]
+    mx, my, ny, nx = [np.asarray(a, dtype=np.float64) for a in surrogate.dft_mats(64, 16)]
+    g = torch.Generator().manual_seed(6)
+    v = torch.randn(2, 32, 64, 64, generator=g).to("cuda", torch.bfloat16).float().requires_grad_()
+    x = surrogate.fno_dft_train(1, v, conv)
+    gx = torch.randn(tuple(x.shape), generator=g)
+    x.backward(gx.cuda())
+    vv = v.detach().cpu().double().numpy()
+    assert _rel(x.detach().cpu().numpy(), ((mx @ vv).reshape(2, 32, 32, 128) @ my)) < 1e-2
+    want = mx.T @ (gx.double().numpy() @ my.T).reshape(2, 32, 64, 64)       # Mx^T view(G My^T)
+    assert _rel(v.grad.cpu().numpy(), want) < 1e-2
+
+    y = torch.randn(2, 32, 32, 32, generator=g).cuda().requires_grad_()
+    out = surrogate.fno_dft_train(-1, y, conv)
+    go = torch.randn(tuple(out.shape), generator=g).to(torch.bfloat16)
+    out.backward(go.cuda())
+    yy = y.detach().cpu().double().numpy()
+    assert _rel(out.detach().float().cpu().numpy(), nx @ (yy @ ny).reshape(2, 32, 64, 64)) < 1e-2
+    want_y = (nx.T @ go.double().numpy()).reshape(2, 32, 32, 128) @ ny.T     # view(Nx^T G) Ny^T
+    assert _rel(y.grad.cpu().numpy(), want_y) < 1e-2
+
+
+def test_network_gradients_with_kernel_dfts_match_fp32_products():
+    """One training loss through the network: the parameter gradients with the kernel-backed
+    transforms agree with those of the FP32 matrix-product transforms (bf16 rounding)."""
+    import torch
+
+    from kwave_amd import surrogate, workloads
+
+    x = torch.as_tensor(np.stack([workloads.surrogate_c_map(5_000_000 + i) for i in range(8)]).astype(np.float32)).cuda()
+    grads = []
+    for fused in (True, False):
+        model = surrogate.build_model(seed=0)
+        for conv in model.spec:
+            conv.fused_dft = fused
+        with torch.no_grad():
+            model.mean.normal_(generator=torch.Generator(device="cuda").manual_seed(1))
+        loss = model(x).square().mean()
+        loss.backward()
+        grads.append({n: p.grad.detach().cpu().numpy() for n, p in model.named_parameters() if p.grad is not None})
+    worst = max(_rel(grads[0][k], grads[1][k]) for k in grads[1] if np.linalg.norm(grads[1][k]) > 0)
+    print(f"worst parameter-gradient rel-L2, kernel vs FP32 transforms: {worst:.2e}")
+    assert worst < 3e-2
 
 
 def test_fused_mode_contraction_matches_gather_path():

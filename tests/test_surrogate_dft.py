@@ -53,3 +53,57 @@ def test_round_trip_keeps_band_limited_field():
     v = np.fft.irfft2(full, s=(n, n))
     back = _inverse(n, md, _apply(n, md, v))
     assert np.abs(back - v).max() < 1e-5 * np.abs(v).max()
+
+
+def _kernel_fwd(v, m1, m2):
+    """The operation kw_fno_dft_bf16 direction 1 performs (without the bf16 roundings):
+    T = m1 v (64 x 64), X = T' m2^T with T' = T viewed 32 x 128."""
+    return (m1 @ v).reshape(v.shape[:-2] + (32, 128)) @ m2.T
+
+
+def _kernel_inv(y, m1, m2):
+    """Direction -1: C = y m1^T (32 x 128), out = m2 C' with C' = C viewed 64 x 64."""
+    return m2 @ (y @ m1.T).reshape(y.shape[:-2] + (64, 64))
+
+
+def test_training_gradients_are_the_adjoint_kernel_calls():
+    """The surrogate's training backward runs the opposite-direction kernel with transposed
+    matrices: direction 1's gradient is direction -1 with (My, Mx^T), direction -1's is
+    direction 1 with (Nx^T, Ny).  Checked as adjoints, <G, F v> = <F* G, v>, in FP64."""
+    rng = np.random.default_rng(2)
+    mx, my, ny, nx = [a.astype(np.float64) for a in dft_mats(64, 16)]
+    v, g = rng.standard_normal((3, 64, 64)), rng.standard_normal((3, 32, 32))
+    lhs = np.sum(g * _kernel_fwd(v, mx, my.T))
+    rhs = np.sum(_kernel_inv(g, my, mx.T) * v)
+    assert abs(lhs - rhs) < 1e-9 * abs(lhs)
+    # the matrices the inference path passes reproduce the transforms themselves
+    assert np.allclose(_kernel_fwd(v[0], mx, my.T)[:, 0::2] + 1j * _kernel_fwd(v[0], mx, my.T)[:, 1::2],
+                       _apply(64, 16, v[0]))
+    y, go = rng.standard_normal((3, 32, 32)), rng.standard_normal((3, 64, 64))
+    lhs = np.sum(go * _kernel_inv(y, ny.T, nx))
+    rhs = np.sum(_kernel_fwd(go, nx.T, ny) * y)
+    assert abs(lhs - rhs) < 1e-9 * abs(lhs)
+    yc = y[0][:, 0::2] + 1j * y[0][:, 1::2]
+    assert np.allclose(_kernel_inv(y[0], ny.T, nx), _inverse(64, 16, yc))
+    # a plausible slip, Mx where Mx^T belongs, is not the adjoint
+    lhs = np.sum(g * _kernel_fwd(v, mx, my.T))
+    assert abs(lhs - np.sum(_kernel_inv(g, my, mx) * v)) > 1e-3 * abs(lhs)
+
+
+def test_routing_threshold_is_the_largest_within_the_miss_budget():
+    from kwave_amd.surrogate import routing_threshold
+
+    rng = np.random.default_rng(3)
+    for trial in range(200):
+        n = int(rng.integers(5, 40))
+        pred = rng.random(n)
+        err = np.where(rng.random(n) < 0.3, 0.05, 0.01) * (1 + rng.random(n))
+        budget = float(rng.choice([0.0, 0.05, 0.1]))
+        t = routing_threshold(pred, err, 0.02, budget)
+        accepted = pred <= t
+        assert np.sum((err > 0.02) & accepted) <= budget * n
+        # brute force: no larger cut over the sorted predictions stays within the budget
+        best = max([k for k in range(n + 1)
+                    if np.sum(err[np.argsort(pred, kind="stable")][:k] > 0.02) <= budget * n])
+        assert accepted.sum() == best
+    assert routing_threshold(np.array([0.1, 0.2]), np.array([0.01, 0.01]), 0.02, 0.0) == 0.2
