@@ -245,6 +245,21 @@ int kw_fno_pointwise_bf16(int B, int C, int64_t P, const void* v, const void* s,
 int kw_spectral_apply(int ndim, const int n[3], const double spacing[3], int use_sg, double kap, int kind, int axis,
                       double power, const float* f, float* out, int device);
 
+/* Allocation-free variant of kw_spectral_apply: the same operator
+ * real(ifftn(op * fftn(f))) with the same kernels (bit-identical results), for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  f and out are device arrays of the grid size on the calling
+ * thread's current device; work is enqueued on `stream` (a hipStream_t, NULL for
+ * the default stream) and the call returns once it is done.  The half spectrum
+ * and the operator tables live in a fixed 64 MiB device array per GPU, held
+ * until the process ends: grids whose spectrum and tables exceed it are refused
+ * with KW_EINVAL before any device work (e.g. 3-D 192^3 fits, 256^3 does not).
+ * Calls that overlap on one GPU share that array and race.  No hipMalloc,
+ * hipFree or other allocating routine is called. */
+int kw_spectral_apply_static(int ndim, const int n[3], const double spacing[3], int use_sg, double kap, int kind, int axis,
+                             double power, const float* f, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,8 @@ EXPORTS = (
     "kw_plan_set_sensor", "kw_plan_init", "kw_plan_run", "kw_plan_read", "kw_plan_field_size",
     "kw_plan_destroy", "kw_spectral_apply", "kw_plan_run_profiled", "kw_plan_launch", "kw_plan_sync", "kw_kernel_kind_count", "kw_kernel_kind_name",
     "kw_plan_adjoint_c", "kw_mode_gemm_bf16", "kw_mode_contract_bf16", "kw_plan_tiles", "kw_fno_pointwise_bf16",
-    "kw_fno_dft_bf16", "kw_fno_head_gather_bf16", "kw_batch_create", "kw_batch_run", "kw_batch_destroy",
+    "kw_fno_dft_bf16", "kw_fno_head_gather_bf16", "kw_spectral_apply_static", "kw_batch_create", "kw_batch_run",
+    "kw_batch_destroy",
 )
 
 
@@ -95,6 +96,7 @@ def lib():
         "kw_plan_field_size": ([vp, i32, i32, C.POINTER(C.c_int64)], i32),
         "kw_plan_destroy": ([vp], i32),
         "kw_spectral_apply": ([i32, C.POINTER(C.c_int), f64p, i32, C.c_double, i32, i32, C.c_double, f32p, f32p, i32], i32),
+        "kw_spectral_apply_static": ([i32, C.POINTER(C.c_int), f64p, i32, C.c_double, i32, i32, C.c_double, vp, vp, vp], i32),
         "kw_plan_run_profiled": ([vp, i64, f64p, f64p, i64p], i32),
         "kw_plan_adjoint_c": ([vp, f32p, i32, f32p], i32),
         "kw_plan_tiles": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], i32),

@@ -19,7 +19,7 @@ LIB = OUT_DIR / "libkwgpu.so"
 # launcher files compiled once per line-length family (-DKW_GROUP=g), heaviest first
 GROUPED = ["kw_zpass_mass.hip", "kw_col_pass.hip", "kw_zpass_mom.hip", "kw_zpass_eos.hip", "kw_zpass_io.hip", "kw_adjoint.hip"]
 GROUPS = (3, 2, 1, 0)
-SOURCES = ["kw_generic.hip", "kw_dispatch.hip", "kw_misc.hip", "kw_abi.hip", "kw_fno.hip"]
+SOURCES = ["kw_generic.hip", "kw_dispatch.hip", "kw_misc.hip", "kw_abi.hip", "kw_fno.hip", "kw_static.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KW_OFFLOAD_ARCH", "gfx950")
