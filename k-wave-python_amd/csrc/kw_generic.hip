@@ -331,7 +331,7 @@ __global__ __launch_bounds__(kGenThreads) void k_gcol(Geom g, Fields fl, Ops ops
         kyz2 = ky * ky;
         if (g.nd == 3) { const float kz = gp(ops.k[2])[iz]; kyz2 += kz * kz; }
       }
-      const float km = sqrtf(fmaf(kx, kx, kyz2));
+      const float km2 = fmaf(kx, kx, kyz2);
       float2 op = make_float2(1.f, 0.f);
       if (jb.kind == XK_GRAD || jb.kind == XK_DIV || jb.kind == XK_UNSTAGGER) {
         const float2* t = jb.kind == XK_GRAD ? ops.grad[jb.axis] : jb.kind == XK_DIV ? ops.div[jb.axis] : ops.unst[jb.axis];
@@ -340,10 +340,10 @@ __global__ __launch_bounds__(kGenThreads) void k_gcol(Geom g, Fields fl, Ops ops
       }
       switch (jb.kind) {
         case XK_GRAD:
-        case XK_DIV: op = cscale(op, kap > 0.f ? sinc_k(kap * km) : 1.f); break;
-        case XK_FRAC1: op = make_float2(kpow(km, ops.frac1), 0.f); break;
-        case XK_FRAC2: op = make_float2(kpow(km, ops.frac2), 0.f); break;
-        case XK_SRCKAPPA: op = make_float2(kap > 0.f ? cos_k(kap * km) : 1.f, 0.f); break;
+        case XK_DIV: op = cscale(op, kap > 0.f ? sinc_k2(kap * kap * km2) : 1.f); break;
+        case XK_FRAC1: op = make_float2(kpow2(km2, ops.frac1), 0.f); break;
+        case XK_FRAC2: op = make_float2(kpow2(km2, ops.frac2), 0.f); break;
+        case XK_SRCKAPPA: op = make_float2(kap > 0.f ? cos_k2(kap * kap * km2) : 1.f, 0.f); break;
         default: break;
       }
       const float2 v = c < nvalid ? ld2(sp, base0 + c + (int64_t)n * a.estride) : make_float2(0.f, 0.f);

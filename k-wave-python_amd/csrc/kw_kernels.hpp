@@ -658,9 +658,15 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_src_build(Geom g, Fields fl, co
 // sin(x)/x and cos(x) by their Taylor series through x^14 (truncation error
 // below 2e-10 for |x| <= 2, which covers c_ref dt |k| / 2 at any stable CFL);
 // larger arguments fall back to the library functions.
-__device__ __forceinline__ float sinc_k(float x) {
-  const float x2 = x * x;
-  if (x2 > 4.f) return sinf(x) / x;
+// The k-space operators of the outermost-axis pass take |k|^2 = kx^2 + ky^2 + kz^2
+// (no square root on the common path).  kappa = sinc(x) and cos(x) for
+// x = c_ref dt |k| / 2 from x^2: Taylor polynomials in x^2 up to x^2 = 4 (FP32
+// rounding), sin(x)/x and cos(x) beyond.
+__device__ __forceinline__ float sinc_k2(float x2) {
+  if (x2 > 4.f) {
+    const float x = sqrtf(x2);
+    return sinf(x) / x;
+  }
   float p = -1.f / 1307674368000.f;                 // -1/15!
   p = fmaf(p, x2, 1.f / 6227020800.f);              //  1/13!
   p = fmaf(p, x2, -1.f / 39916800.f);               // -1/11!
@@ -670,9 +676,8 @@ __device__ __forceinline__ float sinc_k(float x) {
   p = fmaf(p, x2, -1.f / 6.f);                      // -1/3!
   return fmaf(p, x2, 1.f);
 }
-__device__ __forceinline__ float cos_k(float x) {
-  const float x2 = x * x;
-  if (x2 > 4.f) return cosf(x);
+__device__ __forceinline__ float cos_k2(float x2) {
+  if (x2 > 4.f) return cosf(sqrtf(x2));
   float p = -1.f / 87178291200.f;                   // -1/14!
   p = fmaf(p, x2, 1.f / 479001600.f);               //  1/12!
   p = fmaf(p, x2, -1.f / 3628800.f);                // -1/10!
@@ -682,8 +687,8 @@ __device__ __forceinline__ float cos_k(float x) {
   p = fmaf(p, x2, -0.5f);                           // -1/2!
   return fmaf(p, x2, 1.f);
 }
-// |k|^a for |k| > 0 (the fractional Laplacian is 0 at |k| = 0)
-__device__ __forceinline__ float kpow(float km, float a) { return km == 0.f ? 0.f : exp2f(a * log2f(km)); }
+// |k|^a from |k|^2 for |k| > 0 (the fractional Laplacian is 0 at |k| = 0)
+__device__ __forceinline__ float kpow2(float km2, float a) { return km2 == 0.f ? 0.f : exp2f(0.5f * a * log2f(km2)); }
 
 // Consecutive workgroup ids go to different XCDs (round robin over 8); give each
 // XCD a contiguous range of tiles so neighbouring tiles share its L2.
@@ -839,6 +844,7 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COL
     }
     float2 st[C::E];
     int cur_in = -1, pre_in = next_loaded ? a.job[0].in : -1;
+    bool kgroup = false;  // the current input's spectrum carries kappa / ntot
     next_loaded = false;
     for (int j = 0; j < a.njob; ++j) {
       const XJob jb = a.job[j];
@@ -875,6 +881,19 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COL
           next_loaded = true;
         }
         fft_reg<N, -1, typename C::Idx, EM>(st, lds, tw, c, j0, idx);
+        // When every job on this input is a gradient or divergence, the common
+        // factor kappa(|k|) / ntot is applied to the spectrum once (the x
+        // wavenumbers in LDS are published by the transform's barriers).
+        kgroup = true;
+        for (int i = j; i < k; ++i) kgroup &= a.job[i].kind == XK_GRAD || a.job[i].kind == XK_DIV;
+        if (kgroup) {
+#pragma unroll
+          for (int m = 0; m < C::E; ++m) {
+            const float kx = kxs[j0 + m * C::TPL];
+            const float f = kap > 0.f ? sinc_k2(kap * kap * fmaf(kx, kx, kyz2)) : 1.f;
+            st[m] = cscale(st[m], f * ops.inv_ntot);
+          }
+        }
       }
       // where this job's operator values live: an x-table slot, the per-column
       // values, or (a third x-axis table) global memory
@@ -889,16 +908,20 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COL
 #pragma unroll
       for (int m = 0; m < C::E; ++m) {
         const int n = j0 + m * C::TPL;
-        const float kx = kxs[n];
-        const float km = sqrtf(fmaf(kx, kx, kyz2));
         float2 op = make_float2(1.f, 0.f);
         if (tabled) op = jb.axis != 0 ? colv : (slot < 2 ? xop[slot][n] : ld2(gp(gtab), n));
+        if (kgroup) {
+          v[m] = cmul(st[m], op);
+          continue;
+        }
+        const float kx = kxs[n];
+        const float km2 = fmaf(kx, kx, kyz2);
         switch (jb.kind) {
           case XK_GRAD:
-          case XK_DIV: op = cscale(op, kap > 0.f ? sinc_k(kap * km) : 1.f); break;  // kappa(|k|)
-          case XK_FRAC1: op = make_float2(kpow(km, ops.frac1), 0.f); break;
-          case XK_FRAC2: op = make_float2(kpow(km, ops.frac2), 0.f); break;
-          case XK_SRCKAPPA: op = make_float2(kap > 0.f ? cos_k(kap * km) : 1.f, 0.f); break;
+          case XK_DIV: op = cscale(op, kap > 0.f ? sinc_k2(kap * kap * km2) : 1.f); break;  // kappa(|k|)
+          case XK_FRAC1: op = make_float2(kpow2(km2, ops.frac1), 0.f); break;
+          case XK_FRAC2: op = make_float2(kpow2(km2, ops.frac2), 0.f); break;
+          case XK_SRCKAPPA: op = make_float2(kap > 0.f ? cos_k2(kap * kap * km2) : 1.f, 0.f); break;
           default: break;
         }
         v[m] = cscale(cmul(st[m], op), ops.inv_ntot);
