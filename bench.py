@@ -129,6 +129,9 @@ def scan_benchmark(count=96, seed=4):
     t0 = time.perf_counter()
     batched = bs.run()
     t_batched = time.perf_counter() - t0
+    # per size group: grid, instances, the column tile widths and radix plans the plan's timing picked
+    groups = [{"grid": list(bs.setups[idx[0]].shape), "instances": len(idx), "tiles": list(pl.tiles()),
+               "radix_plans": list(pl.radix_plans())} for pl, idx in zip(bs.plans, bs.groups)]
     bs.close()
     log(f"scan: batched {t_batched:.2f} s")
     singles = [DevicePlan([st]) for st in bs.setups]
@@ -142,7 +145,7 @@ def scan_benchmark(count=96, seed=4):
         diff = max(diff, float(np.max(np.abs(one["p"] - res["p"])) / max(np.max(np.abs(one["p"])), 1e-30)))
         pl.close()
     log(f"scan: looped {t_looped:.2f} s, speedup {t_looped / t_batched:.2f}x, max rel diff {diff:.1e}")
-    return {"instances": count, "size_groups": len(bs.groups), "voxel_steps": voxel_steps,
+    return {"instances": count, "size_groups": len(bs.groups), "groups": groups, "voxel_steps": voxel_steps,
             "batched_s": t_batched, "looped_s": t_looped, "speedup_vs_looped": t_looped / t_batched,
             "sims_per_s": count / t_batched, "voxel_steps_per_s": voxel_steps / t_batched,
             "max_rel_diff_batched_vs_looped": diff}
@@ -180,7 +183,7 @@ def surrogate_leg():
     after routing, speed vs the batched solver with and without the fallback."""
     from kwave_amd import surrogate
 
-    res, _ = surrogate.benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=24, log=log)
+    res, _ = surrogate.benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=36, log=log)
     return res
 
 

@@ -163,6 +163,12 @@ const char* kw_kernel_kind_name(int kind);
  * passes with spectral jobs; KW_AUTOTUNE=0 in the environment keeps 0.  No
  * counterpart in the reference (its FFTs are library calls). */
 int kw_plan_tiles(kw_plan* plan, int* fft_tile, int* job_tile);
+/* Radix-plan variants of the same two pass kinds (0: radix 2 next to a factor 5
+ * or 7, 1: radix 4 next to it, fewer stages for more registers; 0 for lengths
+ * without such a factor).  0 unless KW_COL_PLANS in the environment forces them
+ * ("ab") or adds them to the timing ("tune"): the plans agree to FP32 rounding,
+ * not bit for bit.  No counterpart in the reference. */
+int kw_plan_radix_plans(kw_plan* plan, int* fft_plan, int* job_plan);
 /* Copy a device field of instance b to host memory (n floats). */
 int kw_plan_read(kw_plan* plan, int b, int which, float* out, int64_t n);
 /* Number of floats kw_plan_read returns for `which`. */

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/kwgpu.h"
+#include "kw_fft.hpp"
 #include "kw_host.hpp"
 #include "kw_launch.hpp"
 #include "kw_types.hpp"
@@ -102,6 +103,7 @@ struct kw_plan {
   bool graphs = true;
   bool skip_gather = false;  // replayed steps of the reverse sweep do not record
   int col_tile[2] = {0, 0};  // tile variant of the FFT-only and the spectral-job column passes
+  int col_pv[2] = {0, 0};    // radix-plan variant of those passes (lengths with a factor 5 or 7)
   bool tuned = false;
   bool col_wide = false;  // KW_COL_WIDE=1: no narrow-tile fallback on small grids
   int col_persist = 3;    // KW_COL_PERSIST: column passes that loop over tiles (ColArgs::persist)
@@ -206,6 +208,7 @@ int ypass(kw_plan* P, int mode, const std::vector<int>& bufs) {
   if (P->g.nd != 3 || bufs.empty()) return KW_OK;
   ColArgs a = col_args(P->g, 1, mode);
   a.tile = P->col_tile[0];
+  a.pv = P->col_pv[0];
   a.wide = P->col_wide;
   a.persist = P->col_persist;
   a.grid = P->col_grid;
@@ -219,6 +222,7 @@ int ypass(kw_plan* P, int mode, const std::vector<int>& bufs) {
 int xpass(kw_plan* P, const std::vector<XJob>& jobs, int kind) {
   ColArgs a = col_args(P->g, 0, 2);
   a.tile = P->col_tile[1];
+  a.pv = P->col_pv[1];
   a.wide = P->col_wide;
   a.persist = P->col_persist;
   a.grid = P->col_grid;
@@ -696,11 +700,21 @@ int autotune_tiles(kw_plan* P) {
   const char* env = std::getenv("KW_AUTOTUNE");
   if (env && std::atoi(env) == 0) return KW_OK;
   const char* force = std::getenv("KW_COL_TILES");  // two digits, e.g. "10": FFT-only variant, job-pass variant
-  if (force && std::strlen(force) == 2) {
+  // KW_COL_PLANS: two digits force the radix-plan variants (same order); "tune" adds them to
+  // the timed candidates.  By default plan 0 is kept: the plans agree only to FP32 rounding,
+  // and a choice by timing could then differ between a batch and its instances run alone.
+  const char* fplan = std::getenv("KW_COL_PLANS");
+  const bool tune_pv = fplan && std::strcmp(fplan, "tune") == 0;
+  const bool ftile = force && std::strlen(force) == 2, fpv = !tune_pv && fplan && std::strlen(fplan) == 2;
+  if (ftile) {
     P->col_tile[0] = force[0] == '1';
     P->col_tile[1] = force[1] == '1';
-    return KW_OK;
   }
+  if (fpv) {
+    P->col_pv[0] = fplan[0] == '1';
+    P->col_pv[1] = fplan[1] == '1';
+  }
+  if (ftile && fpv) return KW_OK;
   const Geom& g = P->g;
   const int D = g.nd;
   for (int s = 0; s < D; ++s) KW_LAUNCH(launch_fill(reinterpret_cast<float*>(P->fl.spec[s]), 2 * (int64_t)P->nb * g.nspec, 0.f, P->st));
@@ -716,35 +730,53 @@ int autotune_tiles(kw_plan* P) {
     jobs.push_back({0, i, XK_GRAD, i});
     djobs.push_back({i, i, XK_DIV, i});
   }
+  // candidates per pass kind: both tile widths (and, with KW_COL_PLANS=tune, both radix plans
+  // where the length has two); the default (width 0, plan 0) is replaced only by a candidate
+  // >= 3 % faster
   for (int kind = 0; kind < 2 && rc == KW_OK; ++kind) {
-    float t_ms[2] = {0.f, 0.f};
-    for (int tile = 0; tile < 2 && rc == KW_OK; ++tile) {
-      P->col_tile[kind] = tile;
-      auto pass = [&]() -> int {
-        if (kind == 1) {
-          const int r = xpass(P, jobs, KK_XGRAD);
-          return r ? r : xpass(P, djobs, KK_XDIV);
+    const int nline = (kind == 1 || D == 2) ? g.n[0] : g.n[1];
+    const int npv = tune_pv && compiled_size(nline) && has_plan_variant(nline) ? 2 : 1;
+    // a forced choice is the only candidate of its kind
+    const int pv0 = fpv ? P->col_pv[kind] : 0, pv1 = fpv ? P->col_pv[kind] + 1 : npv;
+    const int tl0 = ftile ? P->col_tile[kind] : 0, tl1 = ftile ? P->col_tile[kind] + 1 : 2;
+    float best_ms = -1.f;
+    int best_tile = tl0, best_pv = pv0;
+    for (int pv = pv0; pv < pv1 && rc == KW_OK; ++pv) {
+      for (int tile = tl0; tile < tl1 && rc == KW_OK; ++tile) {
+        P->col_tile[kind] = tile;
+        P->col_pv[kind] = pv;
+        auto pass = [&]() -> int {
+          if (kind == 1) {
+            const int r = xpass(P, jobs, KK_XGRAD);
+            return r ? r : xpass(P, djobs, KK_XDIV);
+          }
+          if (D == 3) return ypass(P, 0, bufs);
+          ColArgs a = col_args(g, 0, 0);
+          a.tile = tile;
+          a.pv = pv;
+          a.persist = P->col_persist;
+          a.grid = P->col_grid;
+          a.nbuf = D;
+          for (int i = 0; i < D; ++i) a.bufs[i] = i;
+          KW_LAUNCH(launch_col(g, P->nb, P->fl, P->ops, P->d_inst, a, g.n[0], P->st));
+          return KW_OK;
+        };
+        if ((rc = pass())) break;  // warm-up
+        if (hipEventRecord(e0, P->st) != hipSuccess) { rc = fail(KW_EHIP, "autotune: event record failed"); break; }
+        for (int r = 0; r < 3 && rc == KW_OK; ++r) rc = pass();
+        if (rc) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, P->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = fail(KW_EHIP, "autotune: timing failed"); break; }
+        if (best_ms < 0.f || ms < 0.97f * best_ms) {
+          best_ms = ms;
+          best_tile = tile;
+          best_pv = pv;
         }
-        if (D == 3) return ypass(P, 0, bufs);
-        ColArgs a = col_args(g, 0, 0);
-        a.tile = tile;
-        a.persist = P->col_persist;
-        a.grid = P->col_grid;
-        a.nbuf = D;
-        for (int i = 0; i < D; ++i) a.bufs[i] = i;
-        KW_LAUNCH(launch_col(g, P->nb, P->fl, P->ops, P->d_inst, a, g.n[0], P->st));
-        return KW_OK;
-      };
-      if ((rc = pass())) break;  // warm-up
-      if (hipEventRecord(e0, P->st) != hipSuccess) { rc = fail(KW_EHIP, "autotune: event record failed"); break; }
-      for (int r = 0; r < 3 && rc == KW_OK; ++r) rc = pass();
-      if (rc) break;
-      float ms = 0.f;
-      if (hipEventRecord(e1, P->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = fail(KW_EHIP, "autotune: timing failed"); break; }
-      t_ms[tile] = ms;
+      }
     }
-    P->col_tile[kind] = (rc == KW_OK && t_ms[1] < 0.97f * t_ms[0]) ? 1 : 0;
+    P->col_tile[kind] = rc == KW_OK ? best_tile : tl0;
+    P->col_pv[kind] = rc == KW_OK ? best_pv : pv0;
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
@@ -758,6 +790,13 @@ int kw_plan_tiles(kw_plan* P, int* fft_tile, int* job_tile) {
   if (!P || !fft_tile || !job_tile) return fail(KW_EINVAL, "NULL argument");
   *fft_tile = P->col_tile[0];
   *job_tile = P->col_tile[1];
+  return KW_OK;
+}
+
+int kw_plan_radix_plans(kw_plan* P, int* fft_plan, int* job_plan) {
+  if (!P || !fft_plan || !job_plan) return fail(KW_EINVAL, "NULL argument");
+  *fft_plan = P->col_pv[0];
+  *job_plan = P->col_pv[1];
   return KW_OK;
 }
 

@@ -45,11 +45,16 @@ struct RadixPlan {
 // next to 3 (E = 12) and radix 2 next to 5 or 7 (E = 10, 14): a few more LDS
 // exchanges, but register arrays short enough for several waves per SIMD (radix
 // 8 gave E = 24 / 40 / 56 and single-wave kernels that spilled).
-constexpr RadixPlan make_plan(int n) {
+//
+// Plan variant pv = 1 (lengths with a factor 5 or 7 only): radix 4 next to 5 or 7
+// (E = 20, 28) - fewer stages, and so fewer LDS exchanges, for twice the
+// registers; the column passes time both variants per plan and keep the faster.
+constexpr RadixPlan make_plan(int n, int pv = 0) {
   RadixPlan p{n, 0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
   int odd = n;
   while (odd > 1 && odd % 2 == 0) odd /= 2;
-  const int p2max = odd == 1 ? 8 : (odd % 5 == 0 || odd % 7 == 0) ? 2 : 4;
+  const bool f57 = odd % 5 == 0 || odd % 7 == 0;
+  const int p2max = odd == 1 ? 8 : f57 ? (pv == 1 ? 4 : 2) : 4;
   int m = n;
   while (p2max >= 8 && m % 8 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 8; m /= 8; }
   while (p2max >= 4 && m % 4 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 4; m /= 4; }
@@ -60,18 +65,24 @@ constexpr RadixPlan make_plan(int n) {
   if (m != 1) p.ns = -1;
   return p;
 }
+// lengths whose plan variant 1 differs from variant 0
+constexpr bool has_plan_variant(int n) {
+  int odd = n;
+  while (odd > 1 && odd % 2 == 0) odd /= 2;
+  return (odd % 5 == 0 || odd % 7 == 0) && n % 4 == 0;
+}
 constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
-constexpr int plan_stage_radix(int n, int s) { return make_plan(n).r[s]; }
-constexpr int plan_nstages(int n) { return make_plan(n).ns; }
-constexpr int plan_ns_before(int n, int s) {
+constexpr int plan_stage_radix(int n, int s, int pv = 0) { return make_plan(n, pv).r[s]; }
+constexpr int plan_nstages(int n, int pv = 0) { return make_plan(n, pv).ns; }
+constexpr int plan_ns_before(int n, int s, int pv = 0) {
   int v = 1;
-  for (int i = 0; i < s; ++i) v *= make_plan(n).r[i];
+  for (int i = 0; i < s; ++i) v *= make_plan(n, pv).r[i];
   return v;
 }
-constexpr int plan_elems(int n) {  // lcm of the radices
+constexpr int plan_elems(int n, int pv = 0) {  // lcm of the radices
   int e = 1;
-  for (int i = 0; i < make_plan(n).ns; ++i) {
-    const int r = make_plan(n).r[i];
+  for (int i = 0; i < make_plan(n, pv).ns; ++i) {
+    const int r = make_plan(n, pv).r[i];
     e = e / gcd_c(e, r) * r;
   }
   return e;
@@ -201,26 +212,26 @@ struct RowIdx {
 };
 
 // EM multiplies the per-thread element count (more loads in flight per thread,
-// fewer threads per line); N must stay divisible by E.
-template <int N, int EM = 1>
+// fewer threads per line); N must stay divisible by E.  PV selects the plan variant.
+template <int N, int EM = 1, int PV = 0>
 struct FftShape {
-  static constexpr int E = plan_elems(N) * EM;
+  static constexpr int E = plan_elems(N, PV) * EM;
   static constexpr int TPL = N / E;
-  static constexpr int S = plan_nstages(N);
+  static constexpr int S = plan_nstages(N, PV);
   static_assert(N % E == 0, "elements per thread must divide the line length");
 };
 
 // One stage on the owned elements v (see header); Idx maps (n, line) -> LDS slot.
-template <int N, int EM, int S, int DIR, class Idx>
-__device__ __forceinline__ void reg_stage(float2 (&v)[(FftShape<N, EM>::E)], float2* lds, const float2* __restrict__ tw, int c, int j0,
-                                          const Idx& idx) {
-  constexpr int E = FftShape<N, EM>::E;
-  constexpr int TPL = FftShape<N, EM>::TPL;
-  constexpr int R = plan_stage_radix(N, S);
-  constexpr int NS = plan_ns_before(N, S);
+template <int N, int EM, int S, int DIR, class Idx, int PV = 0>
+__device__ __forceinline__ void reg_stage(float2 (&v)[(FftShape<N, EM, PV>::E)], float2* lds, const float2* __restrict__ tw, int c,
+                                          int j0, const Idx& idx) {
+  constexpr int E = FftShape<N, EM, PV>::E;
+  constexpr int TPL = FftShape<N, EM, PV>::TPL;
+  constexpr int R = plan_stage_radix(N, S, PV);
+  constexpr int NS = plan_ns_before(N, S, PV);
   constexpr int Q = E / R;
   constexpr int TWS = N / (NS * R);
-  constexpr bool LAST = (S == plan_nstages(N) - 1);
+  constexpr bool LAST = (S == plan_nstages(N, PV) - 1);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = j0 + q * TPL;
@@ -254,23 +265,23 @@ __device__ __forceinline__ void reg_stage(float2 (&v)[(FftShape<N, EM>::E)], flo
   }
 }
 
-template <int N, int EM, int S, int DIR, class Idx>
-__device__ __forceinline__ void reg_stages(float2 (&v)[(FftShape<N, EM>::E)], float2* lds, const float2* __restrict__ tw, int c, int j0,
-                                           const Idx& idx) {
-  if constexpr (S < plan_nstages(N)) {
-    reg_stage<N, EM, S, DIR>(v, lds, tw, c, j0, idx);
-    reg_stages<N, EM, S + 1, DIR>(v, lds, tw, c, j0, idx);
+template <int N, int EM, int S, int DIR, class Idx, int PV = 0>
+__device__ __forceinline__ void reg_stages(float2 (&v)[(FftShape<N, EM, PV>::E)], float2* lds, const float2* __restrict__ tw, int c,
+                                           int j0, const Idx& idx) {
+  if constexpr (S < plan_nstages(N, PV)) {
+    reg_stage<N, EM, S, DIR, Idx, PV>(v, lds, tw, c, j0, idx);
+    reg_stages<N, EM, S + 1, DIR, Idx, PV>(v, lds, tw, c, j0, idx);
   }
 }
 
 // Transform the owned elements of line c in place (natural order in, natural
 // order out at the same owned positions).  Every thread of the workgroup must
 // call it (it contains barriers); the LDS buffer must not be in use.
-template <int N, int DIR, class Idx, int EM = 1>
-__device__ __forceinline__ void fft_reg(float2 (&v)[(FftShape<N, EM>::E)], float2* lds, const float2* __restrict__ tw, int c, int j0,
-                                        const Idx& idx) {
-  static_assert(plan_nstages(N) > 0, "unsupported FFT length");
-  reg_stages<N, EM, 0, DIR>(v, lds, tw, c, j0, idx);
+template <int N, int DIR, class Idx, int EM = 1, int PV = 0>
+__device__ __forceinline__ void fft_reg(float2 (&v)[(FftShape<N, EM, PV>::E)], float2* lds, const float2* __restrict__ tw, int c,
+                                        int j0, const Idx& idx) {
+  static_assert(plan_nstages(N, PV) > 0, "unsupported FFT length");
+  reg_stages<N, EM, 0, DIR, Idx, PV>(v, lds, tw, c, j0, idx);
 }
 
 }  // namespace kw

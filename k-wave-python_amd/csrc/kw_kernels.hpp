@@ -698,9 +698,9 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   return (bid & 7) * (nfull >> 3) + (bid >> 3);
 }
 
-template <int N, int L, int EM = 1>
+template <int N, int L, int EM = 1, int PV = 0>
 struct ColCfg {
-  using F = FftShape<N, EM>;
+  using F = FftShape<N, EM, PV>;
   static constexpr int E = F::E;
   static constexpr int TPL = F::TPL;
   static constexpr int NT = TPL * L;
@@ -718,9 +718,9 @@ constexpr int col_lsmall() { return col_lbig<N>() / 4 < 1 ? 1 : col_lbig<N>() / 
 template <int N>
 constexpr int col_lops() { return col_lbig<N>() / 2 < 1 ? 1 : col_lbig<N>() / 2; }
 
-template <int N, int L, bool OPS, int EM = 1>
-__global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COLOPS : KW_WPE_COL)) void k_col(Geom g, Fields fl, Ops ops, const Inst* __restrict__ inst, ColArgs a) {
-  using C = ColCfg<N, L, EM>;
+template <int N, int L, bool OPS, int EM = 1, int PV = 0>
+__global__ __launch_bounds__((ColCfg<N, L, EM, PV>::NT)) KW_WPE(N, (OPS ? KW_WPE_COLOPS : KW_WPE_COL)) void k_col(Geom g, Fields fl, Ops ops, const Inst* __restrict__ inst, ColArgs a) {
+  using C = ColCfg<N, L, EM, PV>;
   __shared__ float2 lds[C::LDS];
   __shared__ float2 twl[N];
   const typename C::Idx idx{};
@@ -773,8 +773,8 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COL
 #pragma unroll
           for (int m = 0; m < C::E; ++m) nx[m] = ld2(bq, bn + (int64_t)(j0 + m * C::TPL) * a.estride);
         }
-        if (a.mode == 0) fft_reg<N, -1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
-        else fft_reg<N, +1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
+        if (a.mode == 0) fft_reg<N, -1, typename C::Idx, EM, PV>(v, lds, tw, c, j0, idx);
+        else fft_reg<N, +1, typename C::Idx, EM, PV>(v, lds, tw, c, j0, idx);
         if (vt) {
 #pragma unroll
           for (int m = 0; m < C::E; ++m) st2(bp, bt + (int64_t)(j0 + m * C::TPL) * a.estride, v[m]);
@@ -880,7 +880,7 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COL
           for (int m = 0; m < C::E; ++m) nx[PRE ? m : 0] = validn ? ld2(nsrc, basen + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
           next_loaded = true;
         }
-        fft_reg<N, -1, typename C::Idx, EM>(st, lds, tw, c, j0, idx);
+        fft_reg<N, -1, typename C::Idx, EM, PV>(st, lds, tw, c, j0, idx);
         // When every job on this input is a gradient or divergence, the common
         // factor kappa(|k|) / ntot is applied to the spectrum once (the x
         // wavenumbers in LDS are published by the transform's barriers).
@@ -926,7 +926,7 @@ __global__ __launch_bounds__((ColCfg<N, L, EM>::NT)) KW_WPE(N, (OPS ? KW_WPE_COL
         }
         v[m] = cscale(cmul(st[m], op), ops.inv_ntot);
       }
-      fft_reg<N, +1, typename C::Idx, EM>(v, lds, tw, c, j0, idx);
+      fft_reg<N, +1, typename C::Idx, EM, PV>(v, lds, tw, c, j0, idx);
       if (valid) {
         const auto dst = gw(fl.spec[jb.out]);
 #pragma unroll

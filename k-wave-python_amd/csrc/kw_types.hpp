@@ -104,6 +104,7 @@ struct ColArgs {
   int wide;         // 1: keep the chosen width on small grids too (parity tests of the large-grid tiles)
   int persist;      // passes whose workgroups may loop over tiles: bit 0 with jobs, bit 1 FFT-only
   int grid;         // > 0: workgroups per instance when looping (tests of the loop on small grids)
+  int pv;           // radix-plan variant of the transform (lengths with a factor 5 or 7; chosen per plan by timing)
 };
 
 struct Fields {

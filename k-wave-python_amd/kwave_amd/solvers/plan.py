@@ -155,6 +155,13 @@ class DevicePlan:
         check(_abi.lib().kw_plan_tiles(self._h, C.byref(f), C.byref(j)))
         return f.value, j.value
 
+    def radix_plans(self):
+        """(FFT-only, spectral-job) radix-plan variants the plan picked by timing (0 default, 1 radix 4
+        next to a factor 5 or 7)."""
+        f, j = C.c_int(0), C.c_int(0)
+        check(_abi.lib().kw_plan_radix_plans(self._h, C.byref(f), C.byref(j)))
+        return f.value, j.value
+
     def step_index(self, b=0):
         return int(self.read(b, _abi.F_STEP)[0])
 

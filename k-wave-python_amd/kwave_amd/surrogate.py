@@ -531,20 +531,37 @@ def train(model, cmaps, traces, epochs=40, batch=64, lr=2e-3, log=None):
     return hist
 
 
-def predict(model, cmaps, batch=512, features=False):
-    """Surrogate traces (n, 32, 600) in Pa (pointwise layers and head in bf16 autocast); with
-    ``features`` also the error head's inputs (n, 2 width + 3) as float32."""
+def predict(model, cmaps, batch=512, features=False, tta=False):
+    """Surrogate traces (n, 32, 600) in Pa (pointwise layers and head in bf16 autocast).
+
+    ``tta``: also run the map mirrored across the source line (y -> n - 1 - y; the
+    acquisition is symmetric, so the solver's traces of the mirrored map are the
+    traces in reverse sensor order, to ~5e-8) and average the two predictions.
+    ``features``: also return the error head's inputs (n, F) float32 - the forward's
+    features, plus with ``tta`` log10 of the relative disagreement of the two
+    predictions."""
     import torch
 
     dev = next(model.parameters()).device
     out, fts = [], []
     with torch.no_grad(), torch.autocast("cuda", dtype=torch.bfloat16):
         for i in range(0, len(cmaps), batch):
-            r = model(torch.as_tensor(cmaps[i:i + batch], device=dev), features=features)
+            x = torch.as_tensor(cmaps[i:i + batch], device=dev)
+            nb = len(x)
+            r = model(torch.cat([x, x.flip(-1)]) if tta else x, features=features)
             if features:
                 r, f = r
-                fts.append(f.float())
-            out.append(r.float())
+                f = f.float()
+            r = r.float()
+            if tta:
+                a, m = r[:nb], r[nb:].flip(1)
+                r = 0.5 * (a + m)
+                if features:
+                    dis = (a - m).flatten(1).norm(dim=1) / r.flatten(1).norm(dim=1).clamp_min(1e-30)
+                    f = torch.cat([f[:nb], torch.log10(dis.clamp_min(1e-12))[:, None]], 1)
+            out.append(r)
+            if features:
+                fts.append(f)
     traces = (torch.cat(out) * TRACE_SCALE).cpu().numpy()
     return (traces, torch.cat(fts).cpu().numpy()) if features else traces
 
@@ -630,13 +647,13 @@ class Surrogate:
     the solver.
     """
 
-    def __init__(self, model, max_err=0.02):
-        self.model, self.max_err, self.enabled = model, max_err, True
+    def __init__(self, model, max_err=0.02, tta=True):
+        self.model, self.max_err, self.enabled, self.tta = model, max_err, True, tta
         self.head, self.threshold = None, None
 
     def validate(self, cmaps, traces=None):
         traces = solve_batch(np.asarray(cmaps, dtype=np.float64)) if traces is None else traces
-        err = rel_l2(predict(self.model, np.asarray(cmaps, dtype=np.float32)), traces)
+        err = rel_l2(predict(self.model, np.asarray(cmaps, dtype=np.float32), tta=self.tta), traces)
         self.enabled = bool(err.mean() <= self.max_err)
         return err
 
@@ -645,7 +662,7 @@ class Surrogate:
         choose the routing threshold on the rest.  Returns the calibration summary."""
         cmaps = np.asarray(cmaps, dtype=np.float32)
         traces = solve_batch(cmaps.astype(np.float64)) if traces is None else traces
-        pred, feats = predict(self.model, cmaps, features=True)
+        pred, feats = predict(self.model, cmaps, features=True, tta=self.tta)
         err = rel_l2(pred, traces)
         nf = int(round(fit_fraction * len(cmaps)))
         if nf < 16 or len(cmaps) - nf < 16:
@@ -669,8 +686,8 @@ class Surrogate:
             to_solver[:] = True
             return None, to_solver
         if self.head is None:
-            return predict(self.model, cmaps), to_solver
-        pred, feats = predict(self.model, cmaps, features=True)
+            return predict(self.model, cmaps, tta=self.tta), to_solver
+        pred, feats = predict(self.model, cmaps, features=True, tta=self.tta)
         to_solver |= predicted_error(self.head, feats) > self.threshold
         return pred, to_solver
 
@@ -685,19 +702,27 @@ class Surrogate:
         return out, to_solver
 
 
-def time_surrogate(model, cmaps, batch=1024, reps=5, head=None):
-    """Device time per sample of the surrogate forward (inputs already on the device); with an
-    error ``head``, the forward with the head's features and the head itself."""
+def time_surrogate(model, cmaps, batch=1024, reps=5, head=None, tta=False):
+    """Device time per sample of the surrogate forward (inputs already on the device); with
+    ``tta`` the forward of the maps and their mirror images and the average; with an error
+    ``head``, also the head's features and the head itself."""
     import torch
 
     x = torch.as_tensor(cmaps, device=next(model.parameters()).device)
     xb = x.repeat((batch + len(x) - 1) // len(x), 1, 1)[:batch].contiguous()
 
     def once():
-        if head is None:
-            return model(xb)
-        _, f = model(xb, features=True)
-        return head(f.float())
+        xx = torch.cat([xb, xb.flip(-1)]) if tta else xb
+        if head is None and not tta:
+            return model(xx)
+        r, f = model(xx, features=True)
+        f = f.float()
+        if tta:
+            a, m = r[:batch].float(), r[batch:].float().flip(1)
+            r = 0.5 * (a + m)
+            dis = (a - m).flatten(1).norm(dim=1) / r.flatten(1).norm(dim=1).clamp_min(1e-30)
+            f = torch.cat([f[:batch], torch.log10(dis.clamp_min(1e-12))[:, None]], 1)
+        return head(f) if head is not None else r
 
     with torch.no_grad(), torch.autocast("cuda", dtype=torch.bfloat16):
         once()
@@ -742,12 +767,14 @@ def _torch_first():
     torch.cuda.init()
 
 
-def benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=24, max_err=0.02, miss_rate=0.005, log=print):
+def benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=36, max_err=0.02, miss_rate=0.001, log=print):
     """Train on solver data, calibrate the per-map fallback, then report held-out error before
     and after routing and the surrogate's speed against the batched solver, with the fallback's
     solver time included.
 
-    Seeds: training 5,000,000 + j, held-out 6,000,000 + j, calibration 7,000,000 + j.
+    Seeds: training 5,000,000 + j, held-out 6,000,000 + j, calibration 7,000,000 + j.  The
+    routed surrogate averages each map's prediction with that of its mirror image (tta) and
+    feeds the disagreement of the two to the error head.
     """
     _torch_first()
     t0 = time.perf_counter()
@@ -760,19 +787,19 @@ def benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=24, max_err=0.02, m
     t0 = time.perf_counter()
     train(model, xtr, ytr, epochs=epochs, log=log)
     t_train = time.perf_counter() - t0
-    pred = predict(model, xte)
-    err = rel_l2(pred, yte)
+    err = rel_l2(predict(model, xte), yte)
+    err_tta = rel_l2(predict(model, xte, tta=True), yte)
     base = rel_l2(np.broadcast_to(ytr.mean(0), yte.shape), yte)
-    sur = Surrogate(model, max_err=max_err)
+    sur = Surrogate(model, max_err=max_err, tta=True)
     cal = sur.calibrate(xca, yca, miss_rate=miss_rate, log=log)
     _, routed = sur.route(xte)
-    after = np.where(routed, 0.0, err)  # routed maps carry the solver's own traces
-    # throughput, inputs resident on the device: surrogate forward (+ error head) vs the batched
-    # solver's time loop; with the fallback, the routed fraction of maps also pays the solver
+    after = np.where(routed, 0.0, err_tta)  # routed maps carry the solver's own traces
+    # throughput, inputs resident on the device: surrogate forward (+ mirror, error head) vs the
+    # batched solver's time loop; with the fallback, the routed fraction also pays the solver
     t_sur = time_surrogate(model, xte)
-    t_sur_h = time_surrogate(model, xte, head=sur.head)
+    t_sur_r = time_surrogate(model, xte, head=sur.head, tta=True)
     t_sol = time_solver(xte.astype(np.float64))
-    t_eff = t_sur_h + routed.mean() * t_sol
+    t_eff = t_sur_r + routed.mean() * t_sol
     # host to host: c maps in host memory -> traces in host memory (the solver incl. its host set-up)
     t0 = time.perf_counter()
     predict(model, xte)
@@ -788,21 +815,23 @@ def benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=24, max_err=0.02, m
            "datagen_s": t_gen, "width": model.width, "layers": model.layers, "modes": MODES, "grid": N_COARSE,
            "test_rel_l2_mean": float(err.mean()), "test_rel_l2_p95": float(np.percentile(err, 95)),
            "test_frac_above_2pct": float((err > max_err).mean()), "mean_trace_baseline_rel_l2": float(base.mean()),
+           "tta_rel_l2_mean": float(err_tta.mean()), "tta_rel_l2_p95": float(np.percentile(err_tta, 95)),
+           "tta_frac_above_2pct": float((err_tta > max_err).mean()),
            "calibration": cal, "routed_frac": float(routed.mean()),
            "after_routing_rel_l2_mean": float(after.mean()), "after_routing_rel_l2_p95": float(np.percentile(after, 95)),
            "after_routing_frac_above_2pct": float((after > max_err).mean()),
            "after_routing_output_rel_l2_max": float(err_out.max()),
-           "surrogate_ms_per_sample": 1e3 * t_sur, "surrogate_with_error_head_ms_per_sample": 1e3 * t_sur_h,
+           "surrogate_ms_per_sample": 1e3 * t_sur, "routed_surrogate_ms_per_sample": 1e3 * t_sur_r,
            "solver_ms_per_sample_batched": 1e3 * t_sol, "speedup_vs_batched_solver": t_sol / t_sur,
            "speedup_with_fallback": t_sol / t_eff,
            "host_to_host_surrogate_ms_per_sample": 1e3 * t_sur_hh,
            "host_to_host_with_fallback_ms_per_sample": 1e3 * t_fb_hh,
            "host_to_host_solver_ms_per_sample": 1e3 * t_sol_hh,
            "host_to_host_speedup": t_sol_hh / t_sur_hh, "host_to_host_speedup_with_fallback": t_sol_hh / t_fb_hh}
-    log(f"surrogate: held-out rel-L2 {err.mean():.4f} (mean-trace baseline {base.mean():.4f}), "
-        f"{100 * (err > max_err).mean():.1f} % above {100 * max_err:.0f} %; routed {100 * routed.mean():.1f} % -> "
-        f"{100 * (after > max_err).mean():.2f} % above; {1e3 * t_sur:.4f} ms/sample vs solver {1e3 * t_sol:.3f} -> "
-        f"{t_sol / t_sur:.0f}x, {t_sol / t_eff:.1f}x with the fallback")
+    log(f"surrogate: held-out rel-L2 {err.mean():.4f} ({err_tta.mean():.4f} with the mirror average; mean-trace "
+        f"baseline {base.mean():.4f}), {100 * (err_tta > max_err).mean():.1f} % above {100 * max_err:.0f} %; routed "
+        f"{100 * routed.mean():.1f} % -> {100 * (after > max_err).mean():.2f} % above; {1e3 * t_sur:.4f} ms/sample vs "
+        f"solver {1e3 * t_sol:.3f} -> {t_sol / t_sur:.0f}x, {t_sol / t_eff:.1f}x routed with the fallback")
     return res, model
 
 
@@ -820,7 +849,7 @@ def _main():
     ap.add_argument("--train", type=int, default=20000)
     ap.add_argument("--test", type=int, default=1000)
     ap.add_argument("--cal", type=int, default=3000)
-    ap.add_argument("--epochs", type=int, default=24)
+    ap.add_argument("--epochs", type=int, default=36)
     args = ap.parse_args()
     _torch_first()
     if not args.forward:

@@ -92,6 +92,31 @@ def test_looping_column_passes_bit_identical(shape, grid, monkeypatch):
         assert np.array_equal(outs[0][k], outs[1][k]), (shape, grid, k)
 
 
+@pytest.mark.parametrize("shape", [(80, 112), (40, 56, 48), (320, 64), (64, 448)])
+def test_radix_plan_variants_agree_with_the_oracle(shape, monkeypatch):
+    """Lengths with a factor 5 or 7 have two radix plans for the column passes (radix 2 or
+    radix 4 next to the odd factor); the plan picks one by timing, KW_COL_PLANS forces it.
+    Both transform the same lines to FP32 rounding (the two runs differ by rounding alone,
+    about 1e-5 after 50 nonlinear power-law steps), and both match the FP64 oracle."""
+    from test_gpu_parity import _compare, _run_oracle
+
+    from kwave_amd.solvers.kspace_solver import Simulation
+
+    kg, med, src, sen = _case(shape, nt=50)
+    pml = tuple(4 for _ in shape)
+    alpha = tuple(2.0 for _ in shape)
+    outs = []
+    for plans in ("00", "11"):
+        monkeypatch.setenv("KW_COL_PLANS", plans)
+        outs.append(Simulation(kg, med, copy.deepcopy(src), sen, pml_size=pml, pml_alpha=alpha, smooth_p0=False).run())
+    for k in outs[0]:
+        rel = float(np.linalg.norm(outs[0][k] - outs[1][k]) / max(np.linalg.norm(outs[0][k]), 1e-30))
+        assert rel < 5e-5, (shape, k, rel)
+    want = _run_oracle(kg, med, copy.deepcopy(src), sen, pml, alpha, smooth_p0=False)
+    _compare(outs[0], want)
+    _compare(outs[1], want)
+
+
 def _source_case(shape, per_point, nt=83, slen=29):
     """Multi-row additive p and u sources whose signals end before nt: one
     signal row (the dense-pattern form) or one row per point (the spectral
