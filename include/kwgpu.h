@@ -108,6 +108,14 @@ int kw_size_supported(int n);
 int kw_plan_create(const kw_plan_desc* desc, kw_plan** out);
 /* Per-instance time step, c_ref, PML profiles; kspace_solver.py:225-231, 363-406. */
 int kw_plan_set_instance(kw_plan* plan, int b, const kw_inst_desc* d);
+/* Share of the initial density p0 / c0^2 that each split density receives at
+ * t = 0 (share[i] >= 0, summing to 1; 1/ndim each unless set).  The reference
+ * divides p0 evenly over its ndim split densities (kspace_solver.py:729-733); a
+ * 1-D problem runs here as a 2-D grid that is uniform along axis 0, with the
+ * whole initial density in the split density of the propagation axis (shares
+ * {0, 1}), so that the sum evolves as the 1-D density does, PML included.  No
+ * counterpart in the reference. */
+int kw_plan_set_density_split(kw_plan* plan, int b, const double share[3]);
 /* Medium and initial-condition arrays, n = 1 (scalar) or the grid size;
  * kspace_solver.py:228-230, 449-550, 647-689. */
 int kw_plan_set_field(kw_plan* plan, int b, int which, const float* data, int64_t n);

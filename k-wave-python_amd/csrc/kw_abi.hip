@@ -487,6 +487,7 @@ int kw_plan_set_instance(kw_plan* P, int b, const kw_inst_desc* d) {
   I.nt = d->nt;
   I.dt = (float)d->dt;
   I.kap = d->use_kspace ? (float)(d->c_ref * d->dt / 2.0) : 0.f;
+  for (int ax = 0; ax < 3; ++ax) I.p0div[ax] = (float)P->g.nd;
   for (int ax = 0; ax < P->g.nd; ++ax) {
     if (!d->pml[ax] || !d->pml_sg[ax]) return fail(KW_EINVAL, "pml tables required for every axis");
     float *a, *s;
@@ -496,6 +497,22 @@ int kw_plan_set_instance(kw_plan* P, int b, const kw_inst_desc* d) {
     I.pmlsg[ax] = s;
   }
   P->inst_set[b] = 1;
+  P->inst_dirty = true;
+  return sync_inst(P);
+}
+
+int kw_plan_set_density_split(kw_plan* P, int b, const double share[3]) {
+  if (!P || !share || b < 0 || b >= P->nb) return fail(KW_EINVAL, "bad plan, instance or shares");
+  if (!P->inst_set[b]) return fail(KW_EINVAL, "kw_plan_set_instance must precede kw_plan_set_density_split");
+  double sum = 0.0;
+  for (int ax = 0; ax < P->g.nd; ++ax) {
+    if (!(share[ax] >= 0.0 && share[ax] <= 1.0)) return fail(KW_EINVAL, "density shares must lie in [0, 1]");
+    sum += share[ax];
+  }
+  if (std::fabs(sum - 1.0) > 1e-9) return fail(KW_EINVAL, "density shares must sum to 1");
+  KW_HIP(hipSetDevice(P->dev));
+  Inst& I = P->h_inst[b];
+  for (int ax = 0; ax < P->g.nd; ++ax) I.p0div[ax] = share[ax] > 0.0 ? (float)(1.0 / share[ax]) : INFINITY;
   P->inst_dirty = true;
   return sync_inst(P);
 }

@@ -602,9 +602,13 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_init_u(Geom g, Fields fl, const
       const int64_t idx = (int64_t)r * N + n;
       gw(fl.u[ax])[off + idx] = 0.5f * ld(I.dtr0[ax], idx, I.dtr0_s) * (h ? gv[m].y : gv[m].x);
       if (ax == 0) {
-        const float rv = gw(fl.p)[off + idx] / (ld(I.c0sq, idx, I.c0sq_s) * (float)g.nd);
+        const float pv = gw(fl.p)[off + idx], c2 = ld(I.c0sq, idx, I.c0sq_s);
         float rs = 0.f;
-        for (int j = 0; j < g.nd; ++j) { gw(fl.rho[j])[off + idx] = rv; rs += rv; }
+        for (int j = 0; j < g.nd; ++j) {
+          const float rv = pv / (c2 * I.p0div[j]);
+          gw(fl.rho[j])[off + idx] = rv;
+          rs += rv;
+        }
         if (fl.rhot) gw(fl.rhot)[off + idx] = rs;
       }
     }

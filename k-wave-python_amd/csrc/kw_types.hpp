@@ -56,6 +56,7 @@ struct Inst {
   // with a scalar alpha the coefficients are derived from c0^2 in registers:
   // tau = tau_c (c0^2)^{(y-1)/2},  eta = eta_c (c0^2)^{y/2}
   float tau_c, eta_c, ym1h, yh;
+  float p0div[3];       // split density i at t = 0 is p0 / (c0^2 p0div[i]): nd each by default
   Src srcp;
   Src srcu[3];
   // sensors

@@ -123,8 +123,8 @@ def kspaceFirstOrder(kgrid, medium, source, sensor=None, *, pml_size=20, pml_alp
         source.p0 = smooth(np.asarray(source.p0, dtype=float).reshape(tuple(int(n) for n in kgrid.N)), restore_max=True)
     from .solvers.kspace_solver import Simulation
 
-    if kgrid.dim == 1:
-        raise NotImplementedError("1-D grids are not available on the GPU path")
+    if backend == "cpp" and kgrid.dim == 1:
+        raise ValueError("1-D grids run with backend='python' (the C++ binaries' files here are 2-D and 3-D)")
     if backend == "cpp":
         result = _run_cpp_route(kgrid, medium, source, sensor, pml_size, pml_alpha, use_sg, use_kspace, dtype, save_only,
                                 data_path, device_num)

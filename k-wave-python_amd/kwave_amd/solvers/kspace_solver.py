@@ -15,6 +15,7 @@ import warnings
 import numpy as np
 
 from .. import _abi
+from . import embed1d
 from .plan import DevicePlan
 from .setup import build_instance
 
@@ -111,6 +112,7 @@ class Simulation:
                 "pml_size=(20,)*kgrid.dim or use kspaceFirstOrder() which handles PML automatically.",
                 UserWarning, stacklevel=2)
         self._device = self._device_index(device)
+        self._embed1d = False
         self._is_setup = False
         self._plan = None
         self._t = 0
@@ -127,14 +129,26 @@ class Simulation:
 
     # -- lifecycle ------------------------------------------------------------
     def setup(self):
-        """Build the host tables and the device plan (kspace_solver.py:205-241)."""
-        self._setup = build_instance(self.kgrid, self.medium, self.source, self.sensor, use_sg=self.use_sg,
-                                     use_kspace=self.use_kspace, smooth_p0=self.smooth_p0,
-                                     pml_size=self._pml_size_override, pml_alpha=self._pml_alpha_override)
+        """Build the host tables and the device plan (kspace_solver.py:205-241).  A 1-D grid
+        runs as a 2 x N grid uniform along axis 0 (solvers/embed1d.py)."""
+        self._embed1d = embed1d.is_1d(self.kgrid)
+        if self._embed1d:
+            kg, med, src, sen, pml, alpha = embed1d.embed(self.kgrid, self.medium, self.source, self.sensor,
+                                                          self._pml_size_override, self._pml_alpha_override)
+            # the reference does not smooth p0 in 1-D (kspace_solver.py:682)
+            self._setup = embed1d.adjust_setup(build_instance(kg, med, src, sen, use_sg=self.use_sg,
+                                                              use_kspace=self.use_kspace, smooth_p0=False,
+                                                              pml_size=pml, pml_alpha=alpha))
+        else:
+            self._setup = build_instance(self.kgrid, self.medium, self.source, self.sensor, use_sg=self.use_sg,
+                                         use_kspace=self.use_kspace, smooth_p0=self.smooth_p0,
+                                         pml_size=self._pml_size_override, pml_alpha=self._pml_alpha_override)
         st = self._setup
         self.grid_shape, self.spacing, self.ndim = st.shape, st.spacing, st.ndim
+        if self._embed1d:
+            self.grid_shape, self.spacing, self.ndim = st.shape[1:], st.spacing[1:], 1
         self.Nt, self.dt, self.c_ref = st.Nt, st.dt, st.c_ref
-        self.pml_sizes = st.pml_sizes
+        self.pml_sizes = st.pml_sizes[1:] if self._embed1d else st.pml_sizes
         self.record = st.record
         self.n_sensor_points = st.n_sensor
         self.record_start_index = st.rec_start
@@ -170,7 +184,8 @@ class Simulation:
         if remaining > 0:
             self._plan.run(remaining)
         self._t = self._plan.step_index(0)
-        return collect_results(self._plan, 0, self._setup, self._dtype)
+        res = collect_results(self._plan, 0, self._setup, self._dtype)
+        return embed1d.unembed(res) if self._embed1d else res
 
     # -- state inspection (reads device memory) ------------------------------
     # Before the first step() the reference's fields are the zeros of
@@ -179,7 +194,8 @@ class Simulation:
     def _state(self, which):
         if self._t == 0:
             return np.zeros(self.grid_shape, dtype=self._dtype)
-        return self._plan.field(0, which).astype(self._dtype)
+        f = self._plan.field(0, which)
+        return (f[embed1d.ROW] if self._embed1d else f).astype(self._dtype)
 
     @property
     def p(self):
@@ -187,10 +203,14 @@ class Simulation:
 
     @property
     def u(self):
+        if self._embed1d:
+            return [self._state(_abi.F_UX + 1)]  # the propagation axis of the embedding
         return [self._state(_abi.F_UX + i) for i in range(self.ndim)]
 
     @property
     def rho_split(self):
+        if self._embed1d:  # the 1-D density is the sum of the embedding's split densities
+            return [(self._state(_abi.F_RHOX) + self._state(_abi.F_RHOX + 1)).astype(self._dtype)]
         return [self._state(_abi.F_RHOX + i) for i in range(self.ndim)]
 
     def close(self):

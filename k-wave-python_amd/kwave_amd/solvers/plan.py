@@ -74,6 +74,9 @@ class DevicePlan:
             inst.pml[ax] = fptr(a)
             inst.pml_sg[ax] = fptr(g)
         check(L.kw_plan_set_instance(self._h, b, C.byref(inst)))
+        if s.p0_split is not None:
+            share = (C.c_double * 3)(*(tuple(s.p0_split) + (0.0,) * 3)[:3])
+            check(L.kw_plan_set_density_split(self._h, b, share))
         self._field(b, _abi.F_C0SQ, np.asarray(s.c0) ** 2)
         self._field(b, _abi.F_RHO0, s.rho0)
         for ax in range(s.ndim):

@@ -178,6 +178,7 @@ class InstanceSetup:
     requested: set = field(default_factory=set)
     record: set = field(default_factory=set)
     rec_start: int = 0
+    p0_split: tuple = None             # share of p0 / c0^2 per split density at t = 0 (None: 1/ndim each)
 
 
 def build_instance(kgrid, medium, source, sensor, *, use_sg=True, use_kspace=True, smooth_p0=True,

@@ -140,7 +140,7 @@ def _closed_form_ivp(p0, c, dt, dx, n):
     return np.real(np.fft.ifftn(np.cos(n * c * K * dt) * np.fft.fftn(p0)))
 
 
-@pytest.mark.parametrize("shape", [(48, 40), (24, 20, 16)])
+@pytest.mark.parametrize("shape", [(96,), (48, 40), (24, 20, 16)])
 def test_oracle_homogeneous_ivp_matches_closed_form(shape):
     from kwave_amd.kgrid import kWaveGrid
     from kwave_amd.kmedium import kWaveMedium
