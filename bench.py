@@ -298,6 +298,9 @@ def main():
                      "alg_bytes_per_launch": by / nl},
         "hbm_frac_step_model": STEP_MODEL_BYTES * rate / (HBM_PEAK_GBS * 1e9),
         "alg_bytes_per_voxel_step": alg_bytes_step / ntot,
+        # the step's own algorithmic bytes (every kernel's count, summed) over the step time
+        "step_achieved_GBps": alg_bytes_step / ntot * rate / 1e9,
+        "hbm_frac_step_algorithmic": alg_bytes_step / ntot * rate / (HBM_PEAK_GBS * 1e9),
         "gpu_ms_per_step": gpu_ms_step,
         "kernels": kernels,
         "column_tiles": dict(zip(("fft_only", "spectral_jobs"), tiles)),

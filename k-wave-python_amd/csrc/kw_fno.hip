@@ -18,6 +18,7 @@ namespace kw {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Operand layout of v_mfma_f32_32x32x8_bf16 for lane l of the wavefront: the A
 // operand holds row l % 32, k = 4 (l / 32) + 0..3; the B operand column l % 32,
@@ -160,7 +161,7 @@ hipError_t launch_fno_pointwise(int B, int C, int64_t P, const uint16_t* v, cons
 // 32 retained kx (rows interleaved re/im) and 16 retained ky:
 //   forward  T = Mx V (64 x 64, rounded to bf16 in LDS), X = T' My with
 //            T' = T viewed 32 x 128 (row j = [T_2j | T_2j+1]); X 32 x 32 fp32,
-//            columns interleaved re/im per ky;
+//            columns interleaved re/im per ky (four 16 x 16 tiles, one per wave);
 //   inverse  C = Y Ny (32 x 128, rounded to bf16 in LDS), out = Nx C' with
 //            C' = C viewed 64 x 64; out bf16.
 // My and Ny are passed transposed (MyT 32 x 128, NyT 128 x 32) so every MFMA
@@ -207,18 +208,22 @@ __global__ __launch_bounds__(256) void k_dft_fwd(const uint16_t* __restrict__ v,
     for (int r = 0; r < 16; ++r) ts[(mt * 32 + 8 * (r >> 2) + kh + (r & 3)) * DN + nt * 32 + i] = (uint16_t)bf16_rne(acc[r]);
   }
   __syncthreads();
-  if (w == 0) {
-    // T' row j = T rows 2j, 2j+1: element (j, q) sits at ts[j * 2 DN + q]
-    f32x16 acc = {};
+  {
+    // T' row j = T rows 2j, 2j+1: element (j, q) sits at ts[j * 2 DN + q].  X = T' My as four
+    // 16 x 16 output tiles, one per wave (v_mfma_f32_16x16x16_bf16: lane l holds row / column
+    // l % 16 and the four k values 4 (l / 16) .. + 3 of each 16-wide k block)
+    const int r16 = l & 15, kq = (l >> 4) * 4;
+    const int mt2 = w >> 1, nt2 = w & 1;
+    f32x4 acc = {};
 #pragma unroll
-    for (int k0 = 0; k0 < 2 * DN; k0 += 8) {
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(ts + i * 2 * DN + k0 + kh);
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(myt + i * 2 * DN + k0 + kh);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    for (int k0 = 0; k0 < 2 * DN; k0 += 16) {
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(ts + (mt2 * 16 + r16) * 2 * DN + k0 + kq);
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(myt + (nt2 * 16 + r16) * 2 * DN + k0 + kq);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
     float* xo = x + slab * DK * DK;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) xo[(8 * (r >> 2) + kh + (r & 3)) * DK + i] = acc[r];
+    for (int r = 0; r < 4; ++r) xo[(mt2 * 16 + kq + r) * DK + nt2 * 16 + r16] = acc[r];
   }
 }
 

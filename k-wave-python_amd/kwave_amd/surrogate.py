@@ -657,9 +657,14 @@ class Surrogate:
         self.enabled = bool(err.mean() <= self.max_err)
         return err
 
-    def calibrate(self, cmaps, traces=None, miss_rate=0.005, fit_fraction=0.6, log=None):
-        """Fit the error head on the first ``fit_fraction`` of these solver-checked maps and
-        choose the routing threshold on the rest.  Returns the calibration summary."""
+    def calibrate(self, cmaps, traces=None, miss_rate=0.005, fit_fraction=0.6, policy="predicted", log=None):
+        """Fit the error head on the first ``fit_fraction`` of these solver-checked maps and set
+        the routing threshold: ``policy="predicted"`` routes maps whose estimated error exceeds
+        ``max_err``; ``policy="budget"`` takes the largest threshold that leaves at most
+        ``miss_rate`` of the remaining maps accepted while above ``max_err``.  Returns the
+        calibration summary (on the maps not used for the fit)."""
+        if policy not in ("predicted", "budget"):
+            raise ValueError(f"policy must be 'predicted' or 'budget', got {policy!r}")
         cmaps = np.asarray(cmaps, dtype=np.float32)
         traces = solve_batch(cmaps.astype(np.float64)) if traces is None else traces
         pred, feats = predict(self.model, cmaps, features=True, tta=self.tta)
@@ -669,9 +674,10 @@ class Surrogate:
             raise ValueError("calibrate needs at least 16 maps in each split")
         self.head = fit_error_head(feats[:nf], err[:nf], log=log)
         pe = predicted_error(self.head, feats[nf:])
-        self.threshold = routing_threshold(pe, err[nf:], self.max_err, miss_rate)
+        self.threshold = (self.max_err if policy == "predicted"
+                          else routing_threshold(pe, err[nf:], self.max_err, miss_rate))
         routed = pe > self.threshold
-        return {"n_fit": nf, "n_threshold": len(cmaps) - nf, "threshold": self.threshold,
+        return {"policy": policy, "n_fit": nf, "n_threshold": len(cmaps) - nf, "threshold": self.threshold,
                 "routed_frac": float(routed.mean()),
                 "miss_frac": float(((err[nf:] > self.max_err) & ~routed).mean()),
                 "log10_err_corr": float(np.corrcoef(np.log10(pe), np.log10(np.maximum(err[nf:], 1e-6)))[0, 1])}
@@ -791,7 +797,11 @@ def benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=36, max_err=0.02, m
     err_tta = rel_l2(predict(model, xte, tta=True), yte)
     base = rel_l2(np.broadcast_to(ytr.mean(0), yte.shape), yte)
     sur = Surrogate(model, max_err=max_err, tta=True)
-    cal = sur.calibrate(xca, yca, miss_rate=miss_rate, log=log)
+    cal_b = sur.calibrate(xca, yca, miss_rate=miss_rate, policy="budget", log=log)
+    _, routed_b = sur.route(xte)
+    after_b = np.where(routed_b, 0.0, err_tta)
+    # the routing the surrogate keeps: maps whose estimated error exceeds max_err go to the solver
+    cal = sur.calibrate(xca, yca, policy="predicted", log=log)
     _, routed = sur.route(xte)
     after = np.where(routed, 0.0, err_tta)  # routed maps carry the solver's own traces
     # throughput, inputs resident on the device: surrogate forward (+ mirror, error head) vs the
@@ -800,6 +810,7 @@ def benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=36, max_err=0.02, m
     t_sur_r = time_surrogate(model, xte, head=sur.head, tta=True)
     t_sol = time_solver(xte.astype(np.float64))
     t_eff = t_sur_r + routed.mean() * t_sol
+    t_eff_b = t_sur_r + routed_b.mean() * t_sol
     # host to host: c maps in host memory -> traces in host memory (the solver incl. its host set-up)
     t0 = time.perf_counter()
     predict(model, xte)
@@ -818,6 +829,9 @@ def benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=36, max_err=0.02, m
            "tta_rel_l2_mean": float(err_tta.mean()), "tta_rel_l2_p95": float(np.percentile(err_tta, 95)),
            "tta_frac_above_2pct": float((err_tta > max_err).mean()),
            "calibration": cal, "routed_frac": float(routed.mean()),
+           "budget_policy": {"calibration": cal_b, "routed_frac": float(routed_b.mean()),
+                             "after_routing_frac_above_2pct": float((after_b > max_err).mean()),
+                             "speedup_with_fallback": None},
            "after_routing_rel_l2_mean": float(after.mean()), "after_routing_rel_l2_p95": float(np.percentile(after, 95)),
            "after_routing_frac_above_2pct": float((after > max_err).mean()),
            "after_routing_output_rel_l2_max": float(err_out.max()),
@@ -828,6 +842,7 @@ def benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=36, max_err=0.02, m
            "host_to_host_with_fallback_ms_per_sample": 1e3 * t_fb_hh,
            "host_to_host_solver_ms_per_sample": 1e3 * t_sol_hh,
            "host_to_host_speedup": t_sol_hh / t_sur_hh, "host_to_host_speedup_with_fallback": t_sol_hh / t_fb_hh}
+    res["budget_policy"]["speedup_with_fallback"] = t_sol / t_eff_b
     log(f"surrogate: held-out rel-L2 {err.mean():.4f} ({err_tta.mean():.4f} with the mirror average; mean-trace "
         f"baseline {base.mean():.4f}), {100 * (err_tta > max_err).mean():.1f} % above {100 * max_err:.0f} %; routed "
         f"{100 * routed.mean():.1f} % -> {100 * (after > max_err).mean():.2f} % above; {1e3 * t_sur:.4f} ms/sample vs "

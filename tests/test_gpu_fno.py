@@ -80,9 +80,11 @@ def test_surrogate_training_beats_baseline_and_falls_back():
 
     xca, yca = surrogate.make_dataset(7_000_000 + np.arange(256))
     sur = surrogate.Surrogate(model)
-    cal = sur.calibrate(xca, yca, miss_rate=0.01)
+    cal = sur.calibrate(xca, yca, miss_rate=0.01, policy="budget")
     print("calibration:", cal)
     assert cal["miss_frac"] <= 0.01 + 1e-12 and sur.threshold is not None
+    cal = sur.calibrate(xca, yca)                       # route maps estimated above 2 %
+    assert cal["policy"] == "predicted" and sur.threshold == sur.max_err
     out, used_solver = sur(xte)
     err = surrogate.rel_l2(out, yte)
     assert np.all(err[used_solver] == 0.0)            # routed maps carry the solver's traces
