@@ -12,7 +12,8 @@
 // S-1 exchanges and global loads/stores go straight from/to registers.
 //
 // Radices 8, 4, 2, 3, 5, 7.  Twiddles come from a table tw[m] = exp(-2 pi i m/N)
-// built in FP64 on the host and stored as FP32.
+// built in FP64 on the host and stored as FP32, rearranged stage by stage
+// (stage_tw_*) into the array the transform reads.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -86,6 +87,29 @@ constexpr int plan_elems(int n, int pv = 0) {  // lcm of the radices
     e = e / gcd_c(e, r) * r;
   }
   return e;
+}
+
+// Stage-ordered twiddle table.  Stage s >= 1 (NS = product of the radices before
+// it, R its radix) multiplies input r of butterfly class k = j mod NS by
+// w^(r k N / (NS R)), w = exp(-2 pi i / N); its R - 1 twiddles of class k sit next
+// to each other at stage_tw_base(s) + k (R - 1) + (r - 1), so a thread addresses a
+// butterfly's twiddles as one base plus immediate offsets.  The stages take
+// N - R0 entries in all (R0 = the first radix).
+template <int N, int PV = 0>
+constexpr int stage_tw_base(int s) { return plan_ns_before(N, s, PV) - plan_stage_radix(N, 0, PV); }
+template <int N, int PV = 0>
+constexpr int stage_tw_size() { return plan_nstages(N, PV) > 1 ? N - plan_stage_radix(N, 0, PV) : 1; }
+// power of w held by entry e of the stage-ordered table
+template <int N, int PV = 0>
+__host__ __device__ constexpr int stage_tw_power(int e) {
+  for (int s = 1; s < plan_nstages(N, PV); ++s) {
+    const int R = plan_stage_radix(N, s, PV), NS = plan_ns_before(N, s, PV), b = stage_tw_base<N, PV>(s);
+    if (e < b + NS * (R - 1)) {
+      const int i = e - b;
+      return (i % (R - 1) + 1) * (i / (R - 1)) * (N / (NS * R));
+    }
+  }
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -183,13 +207,32 @@ __device__ __forceinline__ void dft(float2* v) {
 }
 
 // ---------------------------------------------------------------------------
-// LDS index maps used for the exchanges
+// LDS index maps used for the exchanges.  Each is idx(n, c) for position n of
+// line c, plus lin(k) with idx(n + k, c) = idx(n, c) + lin(k) whenever k is a
+// multiple of A, or n is a multiple of A and k < A; the transform uses it to
+// address a thread's accesses as one base register plus immediate offsets
+// (A = 0: not used, every access computes its own address).
 // ---------------------------------------------------------------------------
-// lines-fastest (column passes: consecutive threads = consecutive columns)
-// Row pitch L + pad.  The 8-wide x-pass tiles had 2.8 LDS bank-conflict cycles
-// per LDS instruction at pad 1; pad 2 measured best of 1, 2, 3 and 5 at 512^3
-// (0-5 % on the x-passes, about the box-to-box variation).  The 16-wide tiles
-// keep 1 (their conflicts are already near zero).
+// LDS bank model behind these maps (it matches the SQ_LDS_BANK_CONFLICT counts
+// measured for the column passes at 512^3): a b64 access of a wave is served 16
+// lanes at a time over 32 four-byte banks, i.e. 16 float2 slots, so a 16-lane
+// group is conflict-free when its 16 slot indices are distinct mod 16.  In the
+// row passes, base-plus-offset addressing let the compiler merge accesses into
+// two-address forms and raised their conflict cycles by a quarter (97e6 to
+// 122e6 per momentum dispatch at 512^3), so the row map keeps A = 0.
+//
+// lines-fastest (column passes: consecutive threads = consecutive columns).
+// A 16-lane group touches 16 / L rows of L lines: consecutive rows (the reads,
+// and the writes of stages after the first) or rows R apart (the first stage's
+// writes, R = its radix).  Tiles of 16 lines or more keep a pitch of L + 1 (one
+// row per group: conflict-free).  Narrower tiles store rows in groups of
+// G = 32 / L with 4 slots after each group: consecutive rows stay contiguous and
+// rows 8 apart are 4 slots apart mod 16, where a pitch of L + 1 or L + 2 put them
+// on the same banks (at 512 with 8-wide tiles: 3.84 conflict cycles per LDS
+// instruction with pitch 10, 0.29 grouped).
+#ifndef KW_COLGRP
+#define KW_COLGRP 1
+#endif
 #ifndef KW_COLPAD8
 #define KW_COLPAD8 2
 #endif
@@ -198,16 +241,47 @@ __device__ __forceinline__ void dft(float2* v) {
 #endif
 template <int L>
 struct ColIdx {
+#if KW_COLGRP
+  static constexpr int G = L >= 16 ? 0 : 32 / L;  // rows per group (a power of two)
+  static constexpr int GS = G >= 32 ? 5 : G >= 16 ? 4 : G >= 8 ? 3 : G >= 4 ? 2 : G >= 2 ? 1 : 0;  // log2 G
+  static_assert(G == 0 || (1 << GS) == G, "rows per group must be a power of two");
+  static constexpr int PAD = G ? 4 : 0;
+  static constexpr int P = G ? L : L + 1;  // row pitch inside a group
+  static constexpr int A = G ? G : 1;
+  __device__ __forceinline__ int operator()(int n, int c) const { return n * P + c + PAD * (n >> GS); }
+  static constexpr int lin(int k) { return k * P + PAD * (k >> GS); }
+  static constexpr int size(int N) { return N * P + (G ? PAD * ((N + G - 1) / G) : 0); }
+#else
   static constexpr int P = L + (L == 8 ? KW_COLPAD8 : L == 4 ? KW_COLPAD4 : 1);  // row pitch
+  static constexpr int A = 1;
   __device__ __forceinline__ int operator()(int n, int c) const { return n * P + c; }
+  static constexpr int lin(int k) { return k * P; }
   static constexpr int size(int N) { return N * P; }
+#endif
 };
-// line-major with one pad slot every 8 elements (row passes: consecutive
-// threads = consecutive positions of one line)
+// line-major (row passes: consecutive threads = consecutive positions of one
+// line; a 16-lane group is 16 consecutive positions j0 + m TPL, or in the first
+// stage's writes positions 8 apart, or in the later stages' writes two runs of 8
+// that are 64 apart), with a pad slot every 8 positions: about 2 conflict cycles
+// per LDS instruction at 512, mostly the stride-8 writes and the k / N - k pair
+// exchange of the real transforms.  KW_ROWSWZ=1 selects, for N a multiple of 16,
+// slot n ^ h(n) with h taking bits 4-6 of n into bits 0-2 and bit 6 into bit 3,
+// which separates all three transform patterns; it left the 512^3 step unchanged
+// (the row passes are not LDS-bound) and made the batched 2-D sweep 8 % slower
+// (its per-access address arithmetic), so it is off.
+#ifndef KW_ROWSWZ
+#define KW_ROWSWZ 0
+#endif
 template <int N>
 struct RowIdx {
-  static constexpr int LS = N + N / 8 + 1;
-  __device__ __forceinline__ int operator()(int n, int c) const { return c * LS + n + (n >> 3); }
+  static constexpr bool SW = KW_ROWSWZ && N % 16 == 0;
+  static constexpr int LS = SW ? N + 8 : N + N / 8 + 1;
+  static constexpr int A = 0;
+  __device__ __forceinline__ int operator()(int n, int c) const {
+    if constexpr (SW) return c * LS + (n ^ (((n >> 4) & 7) | (((n >> 6) & 1) << 3)));
+    else return c * LS + n + (n >> 3);
+  }
+  static constexpr int lin(int k) { return k; }
   static constexpr int size(int L) { return L * LS; }
 };
 
@@ -230,7 +304,6 @@ __device__ __forceinline__ void reg_stage(float2 (&v)[(FftShape<N, EM, PV>::E)],
   constexpr int R = plan_stage_radix(N, S, PV);
   constexpr int NS = plan_ns_before(N, S, PV);
   constexpr int Q = E / R;
-  constexpr int TWS = N / (NS * R);
   constexpr bool LAST = (S == plan_nstages(N, PV) - 1);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -239,10 +312,12 @@ __device__ __forceinline__ void reg_stage(float2 (&v)[(FftShape<N, EM, PV>::E)],
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = v[q + r * Q];
     if constexpr (NS > 1) {
+      constexpr int TB = stage_tw_base<N, PV>(S);
       const int k = j % NS;
+      const float2* twk = tw + TB + k * (R - 1);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        float2 w = tw[r * k * TWS];
+        float2 w = twk[r - 1];
         if (DIR > 0) w.y = -w.y;
         x[r] = cmul(x[r], w);
       }
@@ -253,14 +328,30 @@ __device__ __forceinline__ void reg_stage(float2 (&v)[(FftShape<N, EM, PV>::E)],
       for (int r = 0; r < R; ++r) v[q + r * Q] = x[r];
     } else {
       const int o = (j / NS) * NS * R + (j % NS);
+      // o + r NS splits into idx(o) + lin(r NS) when the steps are multiples of
+      // A, or when NS divides A and A divides NS R (then o mod A = j mod NS and
+      // r NS mod A <= A - NS never carry past A)
+      constexpr int A = Idx::A;
+      if constexpr (A > 0 && (NS % A == 0 || ((NS * R) % A == 0 && A % NS == 0))) {
+        const int b = idx(o, c);
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[idx(o + r * NS, c)] = x[r];
+        for (int r = 0; r < R; ++r) lds[b + Idx::lin(r * NS)] = x[r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[idx(o + r * NS, c)] = x[r];
+      }
     }
   }
   if constexpr (!LAST) {
     __syncthreads();
+    if constexpr (Idx::A > 0 && TPL % Idx::A == 0) {
+      const int b = idx(j0, c);
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = lds[idx(j0 + m * TPL, c)];
+      for (int m = 0; m < E; ++m) v[m] = lds[b + Idx::lin(m * TPL)];
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = lds[idx(j0 + m * TPL, c)];
+    }
     __syncthreads();
   }
 }

@@ -229,17 +229,33 @@ __device__ __forceinline__ void set_lane(float2 (&a)[E], int m, int h, float v) 
 #ifndef KW_WPE_COL
 #define KW_WPE_COL 1
 #endif
+#ifndef KW_COL_NOPRE
+#define KW_COL_NOPRE 0
+#endif
 #ifndef KW_WPE_COLOPS
 #define KW_WPE_COLOPS 4  // the prefetch of the next job input would take it to 130 VGPRs, 3 waves
 #endif
 #define KW_WPE(N, W) __attribute__((amdgpu_waves_per_eu((FftShape<N>::E <= 8 ? (W) : 1))))
 
+// Stage-ordered twiddles (kw_fft.hpp stage_tw_*) of an N-point plan from the
+// natural table src[m] = exp(-2 pi i m / N), one stage at a time so that every
+// index is a compile-time expression of the entry.
+template <int N, int PV, int S = 1>
+__device__ __forceinline__ void fill_stage_tw(float2* dst, const float2* src, int tid, int nt) {
+  if constexpr (S < plan_nstages(N, PV)) {
+    constexpr int R = plan_stage_radix(N, S, PV), NS = plan_ns_before(N, S, PV);
+    constexpr int B = stage_tw_base<N, PV>(S), TWS = N / (NS * R);
+    for (int e = tid; e < NS * (R - 1); e += nt) dst[B + e] = ld2(gp(src), (e % (R - 1) + 1) * (e / (R - 1)) * TWS);
+    fill_stage_tw<N, PV, S + 1>(dst, src, tid, nt);
+  }
+}
+
 #define KW_ZPASS_PROLOGUE                                              \
   using C = ZCfg<N>;                                                   \
   __shared__ float2 lds[C::LDS];                                       \
-  __shared__ float2 twl[N];                                            \
+  __shared__ float2 twl[stage_tw_size<N>()];                           \
   const int tid = threadIdx.x;                                         \
-  for (int i_ = tid; i_ < N; i_ += C::NT) twl[i_] = tw_g[i_];          \
+  fill_stage_tw<N, 0>(twl, tw_g, tid, C::NT);                          \
   __syncthreads();                                                     \
   const float2* tw = twl;                                              \
   const int b = blockIdx.y;                                            \
@@ -726,12 +742,12 @@ template <int N, int L, bool OPS, int EM = 1, int PV = 0>
 __global__ __launch_bounds__((ColCfg<N, L, EM, PV>::NT)) KW_WPE(N, (OPS ? KW_WPE_COLOPS : KW_WPE_COL)) void k_col(Geom g, Fields fl, Ops ops, const Inst* __restrict__ inst, ColArgs a) {
   using C = ColCfg<N, L, EM, PV>;
   __shared__ float2 lds[C::LDS];
-  __shared__ float2 twl[N];
+  __shared__ float2 twl[stage_tw_size<N, PV>()];
   const typename C::Idx idx{};
   const int tid = threadIdx.x;
   const int b = blockIdx.y;
   if (fl.tdev && gp(fl.tdev)[b] >= inst[b].nt) return;
-  for (int i_ = tid; i_ < N; i_ += C::NT) twl[i_] = ld2(gp(ops.tw[a.axis]), i_);
+  fill_stage_tw<N, PV>(twl, ops.tw[a.axis], tid, C::NT);
   __syncthreads();
   const int ntile = (a.ncols + L - 1) / L;
   const int lid = xcd_remap(blockIdx.x, gridDim.x);
@@ -743,7 +759,27 @@ __global__ __launch_bounds__((ColCfg<N, L, EM, PV>::NT)) KW_WPE(N, (OPS ? KW_WPE
   const float2* tw = twl;
 
   float2 v[C::E];
-  if constexpr (!OPS) {
+  if constexpr (!OPS && KW_COL_NOPRE) {
+    // one buffer of one tile at a time: load, transform, store (latency hidden by
+    // the other workgroups of the CU)
+    const int ntiles = a.nouter * ntile;
+    for (int lt = lid; lt < ntiles; lt += (int)gridDim.x) {
+      const int ot = lt / ntile, colt = (lt % ntile) * L + c;
+      const bool vt = colt < a.ncols;
+      const int64_t bt = (int64_t)b * g.nspec + (int64_t)ot * a.ostride + colt;
+      for (int q = 0; q < a.nbuf; ++q) {
+        const auto bp = gw(fl.spec[a.bufs[q]]);
+#pragma unroll
+        for (int m = 0; m < C::E; ++m) v[m] = vt ? ld2(bp, bt + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
+        if (a.mode == 0) fft_reg<N, -1, typename C::Idx, EM, PV>(v, lds, tw, c, j0, idx);
+        else fft_reg<N, +1, typename C::Idx, EM, PV>(v, lds, tw, c, j0, idx);
+        if (vt) {
+#pragma unroll
+          for (int m = 0; m < C::E; ++m) st2(bp, bt + (int64_t)(j0 + m * C::TPL) * a.estride, v[m]);
+        }
+      }
+    }
+  } else if constexpr (!OPS) {
     // software pipeline over the buffers and tiles: the loads of the next buffer
     // (after a tile's last buffer, the first buffer of this workgroup's next tile)
     // are in flight while the current one is transformed and stored
