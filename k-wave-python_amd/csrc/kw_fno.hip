@@ -10,7 +10,9 @@
 // eight at a time with v_mfma_f32_32x32x8_bf16 (FP32 accumulation).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <mutex>
 
 #include "kw_launch.hpp"
 
@@ -89,8 +91,91 @@ __global__ __launch_bounds__(64) void k_mode_contract(int B, int K, int md, int 
   }
 }
 
+// Tiled form for 16 retained ky modes stored next to each other (ky stride 1 on
+// both sides: one (sample, channel, kx) run of modes is 128 contiguous bytes).
+// A workgroup takes one kx row, its 16 modes and 32 batch rows with all channels:
+//   1. the 32 x C x 16 complex inputs are read run by run (16 lanes per 128-byte
+//      run), rounded to bf16 and stored mode-major in LDS (As[ky][b][k]);
+//   2. wavefront w multiplies modes 4w .. 4w+3, the A operand from LDS and the
+//      weights Bt[m][n][k] of each mode from global memory (L2-resident: 8 KB per
+//      mode), into FP32 accumulators;
+//   3. the results go back through LDS (32 output columns at a time, reusing the
+//      input space) and out run by run again.
+// The per-mode kernel above reads and writes one strided 8-byte value per lane.
+template <int K>
+__global__ __launch_bounds__(256) void k_mode_contract16(int B, int NX, const float2* __restrict__ X, int64_t xsb, int64_t xsc,
+                                                         int64_t xsx, const uint16_t* __restrict__ Bt, float2* __restrict__ Y,
+                                                         int64_t ysb, int64_t ysc, int64_t ysx) {
+  constexpr int MD = 16, RB = 32, C = K / 2;
+  constexpr int RP = K + 4;        // bf16 pitch of a batch row in As (8 pad bytes: the MFMA reads spread over the banks)
+  constexpr int KS = RB * RP + 8;  // bf16 pitch of a mode block in As
+  constexpr int NPASS = K / 32;    // 32-column passes of the output
+  constexpr int YP = MD + 1;       // float pitch of one (row, column) run of modes in Ys
+  constexpr int A_BYTES = MD * KS * 2, Y_BYTES = RB * 32 * YP * 4;
+  static_assert(K % 32 == 0 && K <= 64, "2 C must be 32 or 64");
+  __shared__ __align__(16) unsigned char smem[A_BYTES > Y_BYTES ? A_BYTES : Y_BYTES];
+  uint16_t* As = reinterpret_cast<uint16_t*>(smem);
+  float* Ys = reinterpret_cast<float*>(smem);
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+  const int kxi = blockIdx.y;
+  const int row = kxi < MD ? kxi : NX - 2 * MD + kxi;
+  const int b0 = blockIdx.x * RB;
+  for (int e = tid; e < RB * C * MD; e += 256) {
+    const int ky = e % MD, c = (e / MD) % C, bl = e / (MD * C);
+    const int b = b0 + bl;
+    float2 x = make_float2(0.f, 0.f);
+    if (b < B) x = X[(int64_t)b * xsb + (int64_t)c * xsc + (int64_t)row * xsx + ky];
+    const uint32_t packed = (uint32_t)(uint16_t)bf16_rne(x.x) | ((uint32_t)(uint16_t)bf16_rne(x.y) << 16);
+    *reinterpret_cast<uint32_t*>(&As[ky * KS + bl * RP + 2 * c]) = packed;
+  }
+  __syncthreads();
+  f32x16 acc[4][NPASS];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int ky = 4 * w + q, m = kxi * MD + ky;
+    const uint16_t* aa = As + ky * KS + (l & 31) * RP + (l >> 5) * 4;
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) {
+      const uint16_t* bb = Bt + ((int64_t)m * K + p * 32 + (l & 31)) * K + (l >> 5) * 4;
+      acc[q][p] = f32x16{};
+#pragma unroll
+      for (int k = 0; k < K; k += 8) {
+        const bf16x4 av = *reinterpret_cast<const bf16x4*>(aa + k);
+        const bf16x4 bv = *reinterpret_cast<const bf16x4*>(bb + k);
+        acc[q][p] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc[q][p], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NPASS; ++p) {
+    __syncthreads();  // As (first pass) or the previous pass's Ys is no longer read
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int ky = 4 * w + q;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) Ys[((8 * (r >> 2) + 4 * (l >> 5) + (r & 3)) * 32 + (l & 31)) * YP + ky] = acc[q][p][r];
+    }
+    __syncthreads();
+    for (int e = tid; e < RB * 16 * MD; e += 256) {
+      const int ky = e % MD, o = (e / MD) % 16, bl = e / (MD * 16);
+      const int b = b0 + bl;
+      if (b < B)
+        Y[(int64_t)b * ysb + (int64_t)(p * 16 + o) * ysc + (int64_t)row * ysx + ky] =
+            make_float2(Ys[(bl * 32 + 2 * o) * YP + ky], Ys[(bl * 32 + 2 * o + 1) * YP + ky]);
+    }
+  }
+}
+
 hipError_t launch_mode_contract(int B, int K, int md, int NX, const float2* X, const int64_t xs[4], const uint16_t* Bt, float2* Y,
                                 const int64_t ys[4], hipStream_t st) {
+  if (md == 16 && xs[3] == 1 && ys[3] == 1 && (K == 32 || K == 64)) {
+    const dim3 grid((B + 31) / 32, 2 * md);
+    if (K == 64)
+      hipLaunchKernelGGL(k_mode_contract16<64>, grid, dim3(256), 0, st, B, NX, X, xs[0], xs[1], xs[2], Bt, Y, ys[0], ys[1], ys[2]);
+    else
+      hipLaunchKernelGGL(k_mode_contract16<32>, grid, dim3(256), 0, st, B, NX, X, xs[0], xs[1], xs[2], Bt, Y, ys[0], ys[1], ys[2]);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_mode_contract, dim3(K / 32, (B + 31) / 32, 2 * md * md), dim3(64), 0, st, B, K, md, NX, X, xs[0], xs[1],
                      xs[2], xs[3], Bt, reinterpret_cast<float*>(Y), ys[0], ys[1], ys[2], ys[3]);
   return hipGetLastError();
@@ -174,94 +259,127 @@ constexpr int DN = 64, DK = 32;  // slab edge, retained kx rows (= interleaved k
 constexpr int LDT = DN + 4;
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_dft_fwd(const uint16_t* __restrict__ v, const uint16_t* __restrict__ mx,
+// Both kernels run on a resident grid whose workgroups loop over slabs: the
+// fixed DFT-matrix operands of every MFMA stay in registers for the whole loop,
+// and the next slab's input is loaded while the current one is transformed.
+__global__ __launch_bounds__(256) void k_dft_fwd(int64_t S, const uint16_t* __restrict__ v, const uint16_t* __restrict__ mx,
                                                  const uint16_t* __restrict__ myt, float* __restrict__ x) {
   __shared__ __attribute__((aligned(16))) uint16_t vt[DN * LDT];  // the slab, transposed: vt[y][x]
   __shared__ __attribute__((aligned(16))) uint16_t ts[DN * DN];   // T = Mx V
-  const int64_t slab = blockIdx.x;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int i = l & 31, kh = (l >> 5) * 4;
-  {
+  const int mt = w >> 1, nt = w & 1;
+  // T' row j = T rows 2j, 2j+1: element (j, q) sits at ts[j * 2 DN + q].  X = T' My as four
+  // 16 x 16 output tiles, one per wave (v_mfma_f32_16x16x16_bf16: lane l holds row / column
+  // l % 16 and the four k values 4 (l / 16) .. + 3 of each 16-wide k block)
+  const int r16 = l & 15, kq = (l >> 4) * 4;
+  bf16x4 am[DN / 8], bm[2 * DN / 16];
+#pragma unroll
+  for (int k = 0; k < DN / 8; ++k) am[k] = *reinterpret_cast<const bf16x4*>(mx + (mt * 32 + i) * DN + 8 * k + kh);
+#pragma unroll
+  for (int k = 0; k < 2 * DN / 16; ++k) bm[k] = *reinterpret_cast<const bf16x4*>(myt + (nt * 16 + r16) * 2 * DN + 16 * k + kq);
+  constexpr int QL = DN * DN / 8 / 256;  // 16-byte loads per thread and slab
+  uint4 q[QL];
+  int64_t slab = blockIdx.x;
+  if (slab < S) {
     const uint4* src = reinterpret_cast<const uint4*>(v + slab * DN * DN);
-    for (int e = tid; e < DN * DN / 8; e += 256) {
-      const uint4 q = src[e];
+#pragma unroll
+    for (int t = 0; t < QL; ++t) q[t] = src[tid + 256 * t];
+  }
+  for (; slab < S; slab += gridDim.x) {
+    __syncthreads();  // the previous slab's reads of vt and ts are done
+#pragma unroll
+    for (int t = 0; t < QL; ++t) {
+      const int e = tid + 256 * t;
       const int xr = e >> 3, y0 = (e & 7) * 8;
-      const unsigned u[4] = {q.x, q.y, q.z, q.w};
+      const unsigned u[4] = {q[t].x, q[t].y, q[t].z, q[t].w};
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         vt[(y0 + 2 * j) * LDT + xr] = (uint16_t)(u[j] & 0xffffu);
         vt[(y0 + 2 * j + 1) * LDT + xr] = (uint16_t)(u[j] >> 16);
       }
     }
-  }
-  __syncthreads();
-  {
-    const int mt = w >> 1, nt = w & 1;
-    f32x16 acc = {};
+    if (slab + gridDim.x < S) {
+      const uint4* src = reinterpret_cast<const uint4*>(v + (slab + gridDim.x) * DN * DN);
 #pragma unroll
-    for (int k0 = 0; k0 < DN; k0 += 8) {
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(mx + (mt * 32 + i) * DN + k0 + kh);
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(vt + (nt * 32 + i) * LDT + k0 + kh);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+      for (int t = 0; t < QL; ++t) q[t] = src[tid + 256 * t];
     }
+    __syncthreads();
+    {
+      f32x16 acc = {};
 #pragma unroll
-    for (int r = 0; r < 16; ++r) ts[(mt * 32 + 8 * (r >> 2) + kh + (r & 3)) * DN + nt * 32 + i] = (uint16_t)bf16_rne(acc[r]);
-  }
-  __syncthreads();
-  {
-    // T' row j = T rows 2j, 2j+1: element (j, q) sits at ts[j * 2 DN + q].  X = T' My as four
-    // 16 x 16 output tiles, one per wave (v_mfma_f32_16x16x16_bf16: lane l holds row / column
-    // l % 16 and the four k values 4 (l / 16) .. + 3 of each 16-wide k block)
-    const int r16 = l & 15, kq = (l >> 4) * 4;
-    const int mt2 = w >> 1, nt2 = w & 1;
-    f32x4 acc = {};
+      for (int k = 0; k < DN / 8; ++k) {
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(vt + (nt * 32 + i) * LDT + 8 * k + kh);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(am[k], b, acc, 0, 0, 0);
+      }
 #pragma unroll
-    for (int k0 = 0; k0 < 2 * DN; k0 += 16) {
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(ts + (mt2 * 16 + r16) * 2 * DN + k0 + kq);
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(myt + (nt2 * 16 + r16) * 2 * DN + k0 + kq);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+      for (int r = 0; r < 16; ++r) ts[(mt * 32 + 8 * (r >> 2) + kh + (r & 3)) * DN + nt * 32 + i] = (uint16_t)bf16_rne(acc[r]);
     }
-    float* xo = x + slab * DK * DK;
+    __syncthreads();
+    {
+      f32x4 acc = {};
 #pragma unroll
-    for (int r = 0; r < 4; ++r) xo[(mt2 * 16 + kq + r) * DK + nt2 * 16 + r16] = acc[r];
+      for (int k = 0; k < 2 * DN / 16; ++k) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(ts + (mt * 16 + r16) * 2 * DN + 16 * k + kq);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bm[k], acc, 0, 0, 0);
+      }
+      float* xo = x + slab * DK * DK;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xo[(mt * 16 + kq + r) * DK + nt * 16 + r16] = acc[r];
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void k_dft_inv(const float* __restrict__ y, const uint16_t* __restrict__ nyt,
+__global__ __launch_bounds__(256) void k_dft_inv(int64_t S, const float* __restrict__ y, const uint16_t* __restrict__ nyt,
                                                  const uint16_t* __restrict__ nx, uint16_t* __restrict__ out) {
   __shared__ __attribute__((aligned(16))) uint16_t ct[DN * LDT];  // C' = (Y Ny viewed 64 x 64), transposed: ct[n][k]
-  const int64_t slab = blockIdx.x;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int i = l & 31, kh = (l >> 5) * 4;
-  {
-    // wave w: columns [32 w, 32 w + 32) of C, K = 32
+  const int mt = w >> 1, nt = w & 1;
+  bf16x4 bn[DK / 8], an[DN / 8];
+#pragma unroll
+  for (int k = 0; k < DK / 8; ++k) bn[k] = *reinterpret_cast<const bf16x4*>(nyt + (w * 32 + i) * DK + 8 * k + kh);
+#pragma unroll
+  for (int k = 0; k < DN / 8; ++k) an[k] = *reinterpret_cast<const bf16x4*>(nx + (mt * 32 + i) * DN + 8 * k + kh);
+  float4 yq[DK / 8];
+  int64_t slab = blockIdx.x;
+  if (slab < S) {
     const float* ys = y + slab * DK * DK;
-    f32x16 acc = {};
 #pragma unroll
-    for (int k0 = 0; k0 < DK; k0 += 8) {
-      const float4 yv = *reinterpret_cast<const float4*>(ys + i * DK + k0 + kh);
-      const bf16x4 a = bf16x4{bf16_rne(yv.x), bf16_rne(yv.y), bf16_rne(yv.z), bf16_rne(yv.w)};
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(nyt + (w * 32 + i) * DK + k0 + kh);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    // C[j][q] is C'[2 j + q / 64][q % 64], with q = 32 w + i
-    const int n = (w & 1) * 32 + i, kk = w >> 1;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) ct[n * LDT + 2 * (8 * (r >> 2) + kh + (r & 3)) + kk] = (uint16_t)bf16_rne(acc[r]);
+    for (int k = 0; k < DK / 8; ++k) yq[k] = *reinterpret_cast<const float4*>(ys + i * DK + 8 * k + kh);
   }
-  __syncthreads();
-  {
-    const int mt = w >> 1, nt = w & 1;
+  for (; slab < S; slab += gridDim.x) {
+    // wave w: columns [32 w, 32 w + 32) of C, K = 32
     f32x16 acc = {};
 #pragma unroll
-    for (int k0 = 0; k0 < DN; k0 += 8) {
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(nx + (mt * 32 + i) * DN + k0 + kh);
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(ct + (nt * 32 + i) * LDT + k0 + kh);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    for (int k = 0; k < DK / 8; ++k) {
+      const bf16x4 a = bf16x4{bf16_rne(yq[k].x), bf16_rne(yq[k].y), bf16_rne(yq[k].z), bf16_rne(yq[k].w)};
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bn[k], acc, 0, 0, 0);
     }
-    uint16_t* o = out + slab * DN * DN;
+    if (slab + gridDim.x < S) {
+      const float* ys = y + (slab + gridDim.x) * DK * DK;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) o[(mt * 32 + 8 * (r >> 2) + kh + (r & 3)) * DN + nt * 32 + i] = (uint16_t)bf16_rne(acc[r]);
+      for (int k = 0; k < DK / 8; ++k) yq[k] = *reinterpret_cast<const float4*>(ys + i * DK + 8 * k + kh);
+    }
+    __syncthreads();  // the previous slab's reads of ct are done
+    {
+      // C[j][q] is C'[2 j + q / 64][q % 64], with q = 32 w + i
+      const int n = (w & 1) * 32 + i, kk = w >> 1;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) ct[n * LDT + 2 * (8 * (r >> 2) + kh + (r & 3)) + kk] = (uint16_t)bf16_rne(acc[r]);
+    }
+    __syncthreads();
+    {
+      f32x16 acc2 = {};
+#pragma unroll
+      for (int k = 0; k < DN / 8; ++k) {
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(ct + (nt * 32 + i) * LDT + 8 * k + kh);
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(an[k], b, acc2, 0, 0, 0);
+      }
+      uint16_t* o = out + slab * DN * DN;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) o[(mt * 32 + 8 * (r >> 2) + kh + (r & 3)) * DN + nt * 32 + i] = (uint16_t)bf16_rne(acc2[r]);
+    }
   }
 }
 
@@ -291,13 +409,38 @@ hipError_t launch_head_gather(int64_t B, int C, int K, const uint16_t* v, const 
   return hipGetLastError();
 }
 
+// Resident workgroups of a 256-thread kernel on the current device (cached per
+// device and kernel; 0 when the runtime cannot say).
+static int64_t resident_groups(const void* kernel, int slot) {
+  constexpr int kDevices = 64;
+  static std::mutex mu;
+  static int64_t cache[kDevices][2] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevices) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  if (cache[dev][slot] == 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu < 1 || cus < 1)
+      cache[dev][slot] = -1;
+    else
+      cache[dev][slot] = (int64_t)per_cu * cus;
+  }
+  return cache[dev][slot] > 0 ? cache[dev][slot] : 0;
+}
+
+static dim3 slab_grid(int64_t S, const void* kernel, int slot) {
+  const int64_t res = resident_groups(kernel, slot);
+  return dim3((unsigned)(res > 0 ? std::min<int64_t>(S, res) : std::min<int64_t>(S, 2048)));
+}
+
 hipError_t launch_dft_fwd(int64_t S, const uint16_t* v, const uint16_t* mx, const uint16_t* myt, float* x, hipStream_t st) {
-  hipLaunchKernelGGL(k_dft_fwd, dim3((unsigned)S), dim3(256), 0, st, v, mx, myt, x);
+  hipLaunchKernelGGL(k_dft_fwd, slab_grid(S, reinterpret_cast<const void*>(&k_dft_fwd), 0), dim3(256), 0, st, S, v, mx, myt, x);
   return hipGetLastError();
 }
 
 hipError_t launch_dft_inv(int64_t S, const float* y, const uint16_t* nyt, const uint16_t* nx, uint16_t* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_dft_inv, dim3((unsigned)S), dim3(256), 0, st, y, nyt, nx, out);
+  hipLaunchKernelGGL(k_dft_inv, slab_grid(S, reinterpret_cast<const void*>(&k_dft_inv), 1), dim3(256), 0, st, S, y, nyt, nx, out);
   return hipGetLastError();
 }
 

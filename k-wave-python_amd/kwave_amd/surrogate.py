@@ -108,7 +108,9 @@ def mode_contract(x, bt, md):
         raise ValueError("mode_contract takes a complex64 spectrum and contiguous bf16 weights")
     if tuple(bt.shape) != (2 * md * md, 2 * c, 2 * c):
         raise ValueError(f"weights {tuple(bt.shape)} do not match {2 * md * md} modes of {c} channels")
-    y = torch.view_as_complex(torch.zeros((b, c, nx, ny, 2), dtype=torch.float32, device=x.device))
+    # the kernels write every retained mode; only a spectrum with more than those needs zeros elsewhere
+    alloc = torch.empty if (nx == 2 * md and ny == md) else torch.zeros
+    y = torch.view_as_complex(alloc((b, c, nx, ny, 2), dtype=torch.float32, device=x.device))
     xs = (C.c_int64 * 4)(*x.stride())
     ys = (C.c_int64 * 4)(*y.stride())
     stream = torch.cuda.current_stream(x.device).cuda_stream
@@ -531,7 +533,7 @@ def train(model, cmaps, traces, epochs=40, batch=64, lr=2e-3, log=None):
     return hist
 
 
-def predict(model, cmaps, batch=512, features=False, tta=False):
+def predict(model, cmaps, batch=512, features=False, tta=False, ranges=False):
     """Surrogate traces (n, 32, 600) in Pa (pointwise layers and head in bf16 autocast).
 
     ``tta``: also run the map mirrored across the source line (y -> n - 1 - y; the
@@ -539,15 +541,36 @@ def predict(model, cmaps, batch=512, features=False, tta=False):
     traces in reverse sensor order, to ~5e-8) and average the two predictions.
     ``features``: also return the error head's inputs (n, F) float32 - the forward's
     features, plus with ``tta`` log10 of the relative disagreement of the two
-    predictions."""
+    predictions.  ``ranges``: also return each map's (min, max) (n, 2), taken on the
+    device from the uploaded map.
+
+    Host traffic: each batch of maps is copied into one of two page-locked staging
+    buffers and uploaded asynchronously, so that the copy of the next batch overlaps
+    the forward of the current one; the traces come back asynchronously into a
+    page-locked array that is returned as is (no further host copy)."""
     import torch
 
     dev = next(model.parameters()).device
-    out, fts = [], []
+    cm = np.ascontiguousarray(cmaps, dtype=np.float32)
+    n = len(cm)
+    src = torch.from_numpy(cm)
+    stream = torch.cuda.current_stream(dev)
+    stage = [torch.empty((min(batch, n),) + cm.shape[1:], dtype=torch.float32, pin_memory=True) for _ in range(2)]
+    uploaded = [None, None]
+    out, fts, rng = None, [], []
     with torch.no_grad(), torch.autocast("cuda", dtype=torch.bfloat16):
-        for i in range(0, len(cmaps), batch):
-            x = torch.as_tensor(cmaps[i:i + batch], device=dev)
-            nb = len(x)
+        for k, i in enumerate(range(0, n, batch)):
+            nb = min(batch, n - i)
+            buf = stage[k % 2]
+            if uploaded[k % 2] is not None:
+                uploaded[k % 2].synchronize()  # the upload from this buffer two batches ago has finished
+            buf[:nb].copy_(src[i:i + nb])
+            x = buf[:nb].to(dev, non_blocking=True)
+            uploaded[k % 2] = torch.cuda.Event()
+            uploaded[k % 2].record(stream)
+            if ranges:
+                flat = x.reshape(nb, -1)
+                rng.append(torch.stack([flat.amin(1), flat.amax(1)], 1))
             r = model(torch.cat([x, x.flip(-1)]) if tta else x, features=features)
             if features:
                 r, f = r
@@ -559,11 +582,19 @@ def predict(model, cmaps, batch=512, features=False, tta=False):
                 if features:
                     dis = (a - m).flatten(1).norm(dim=1) / r.flatten(1).norm(dim=1).clamp_min(1e-30)
                     f = torch.cat([f[:nb], torch.log10(dis.clamp_min(1e-12))[:, None]], 1)
-            out.append(r)
+            r = r * TRACE_SCALE
+            if out is None:
+                out = torch.empty((n,) + tuple(r.shape[1:]), dtype=torch.float32, pin_memory=True)
+            out[i:i + nb].copy_(r, non_blocking=True)
             if features:
                 fts.append(f)
-    traces = (torch.cat(out) * TRACE_SCALE).cpu().numpy()
-    return (traces, torch.cat(fts).cpu().numpy()) if features else traces
+    stream.synchronize()
+    res = [out.numpy()]
+    if features:
+        res.append(torch.cat(fts).cpu().numpy())
+    if ranges:
+        res.append(torch.cat(rng).cpu().numpy())
+    return res[0] if len(res) == 1 else tuple(res)
 
 
 def build_error_head(n_in, device="cuda", seed=1, hidden=64):
@@ -683,29 +714,33 @@ class Surrogate:
                 "log10_err_corr": float(np.corrcoef(np.log10(pe), np.log10(np.maximum(err[nf:], 1e-6)))[0, 1])}
 
     def route(self, cmaps):
-        """(traces of the surrogate for every map, mask of maps to send to the solver)."""
+        """(traces of the surrogate for every map, mask of maps to send to the solver).
+
+        A map goes to the solver when its sound speed leaves the training range or,
+        with an error head, when the head's estimate of its error exceeds the threshold."""
         cmaps = np.asarray(cmaps, dtype=np.float32)
         lo, hi = workloads.SURROGATE_C_RANGE
-        flat = cmaps.reshape(len(cmaps), -1)
-        to_solver = (flat.min(1) < lo - 1e-3) | (flat.max(1) > hi + 1e-3)
         if not self.enabled:
-            to_solver[:] = True
-            return None, to_solver
+            return None, np.ones(len(cmaps), dtype=bool)
         if self.head is None:
-            return predict(self.model, cmaps, tta=self.tta), to_solver
-        pred, feats = predict(self.model, cmaps, features=True, tta=self.tta)
-        to_solver |= predicted_error(self.head, feats) > self.threshold
+            pred, rng = predict(self.model, cmaps, tta=self.tta, ranges=True)
+        else:
+            pred, feats, rng = predict(self.model, cmaps, features=True, tta=self.tta, ranges=True)
+        to_solver = (rng[:, 0] < lo - 1e-3) | (rng[:, 1] > hi + 1e-3)
+        if self.head is not None:
+            to_solver |= predicted_error(self.head, feats) > self.threshold
         return pred, to_solver
 
     def __call__(self, cmaps):
+        """(traces of every map, mask of the maps the solver ran): the surrogate's traces
+        with the routed maps' rows replaced by the solver's."""
         cmaps = np.asarray(cmaps, dtype=np.float32)
         pred, to_solver = self.route(cmaps)
-        out = np.empty((len(cmaps),) + tuple(self.model.mean.shape), dtype=np.float32)
-        if pred is not None:
-            out[~to_solver] = pred[~to_solver]
+        if pred is None:
+            return solve_batch(cmaps.astype(np.float64)).astype(np.float32), to_solver
         if to_solver.any():
-            out[to_solver] = solve_batch(cmaps[to_solver].astype(np.float64))
-        return out, to_solver
+            pred[to_solver] = solve_batch(cmaps[to_solver].astype(np.float64))
+        return pred, to_solver
 
 
 def time_surrogate(model, cmaps, batch=1024, reps=5, head=None, tta=False):

@@ -153,15 +153,17 @@ def test_network_gradients_with_kernel_dfts_match_fp32_products():
     assert worst < 3e-2
 
 
-def test_fused_mode_contraction_matches_gather_path():
+@pytest.mark.parametrize("b,c", [(40, 16), (70, 32)])
+def test_fused_mode_contraction_matches_gather_path(b, c):
     """kw_mode_contract_bf16 (gather + MFMA + scatter fused) == explicit gather + kw_mode_gemm_bf16 + scatter,
-    forward and gradients, bit for bit (same bf16 operands, same accumulation order)."""
+    forward and gradients, bit for bit (same bf16 operands, same accumulation order).  16 retained
+    modes run the tiled kernel (32 batch rows per workgroup: 40 and 70 leave ragged last tiles)."""
     import torch
 
     from kwave_amd.surrogate import gather_modes, spectral_contract, spectral_modes
 
     torch.manual_seed(3)
-    b, c, n, md = 40, 16, 64, 16
+    n, md = 64, 16
     v = torch.randn(b, c, n, n, device="cuda")
     wr = (0.05 * torch.randn(2 * md * md, c, c, device="cuda")).requires_grad_()
     wi = (0.05 * torch.randn(2 * md * md, c, c, device="cuda")).requires_grad_()
@@ -176,7 +178,8 @@ def test_fused_mode_contraction_matches_gather_path():
     fused = [out.detach().clone(), ft.grad.clone(), wr.grad.clone(), wi.grad.clone()]
     ft.grad = wr.grad = wi.grad = None
 
-    x2 = torch.nn.functional.pad(gather_modes(ft, md), (0, 0, 0, 64 - b))
+    bp = (b + 31) // 32 * 32
+    x2 = torch.nn.functional.pad(gather_modes(ft, md), (0, 0, 0, bp - b))
     y2 = spectral_contract(x2, w2())[:, :b]
     y = torch.view_as_complex(y2.reshape(2 * md, md, b, c, 2)).permute(2, 3, 0, 1)
     kx = torch.cat([torch.arange(md), torch.arange(n - md, n)]).cuda()
