@@ -234,6 +234,15 @@ int kw_mode_contract_bf16(int B, int C, int md, int NX, int NY, const void* X, c
  * `stream`.  No counterpart in the reference (it has no surrogate). */
 int kw_fno_dft_bf16(int dir, int64_t S, const void* in, const void* m1, const void* m2, void* out, void* stream);
 
+/* Input lift of the surrogate for inference: v[b][c][i][j] = bf16(x[b][i][j] lc[c] +
+ * g[c][i][j]) with x the 2 x 2 average of the normalised fine map
+ * (cmap - cmid) / chalf; cmap fp32 [B][2N][2N] (16-byte aligned), lc fp32 [C],
+ * g bf16 [C][N][N] (the lift of the coordinate grid), v bf16 [B][C][N][N]; device
+ * pointers, C 16 or 32, N a multiple of 4.  Enqueued on `stream`.  No counterpart
+ * in the reference (it has no surrogate). */
+int kw_fno_lift_bf16(int64_t B, int C, int N, const float* cmap, float cmid, float chalf, const float* lc, const void* g, void* v,
+                     void* stream);
+
 /* Sensor-head gather of the surrogate for inference: sel[b][k][c][x] =
  * v[b][c][x][cols[k]] for b < B, k < K, c < C, x < 64, with v bf16 [B][C][64][64],
  * cols int32 [K] (device, every value in [0, 64): the caller checks), sel bf16

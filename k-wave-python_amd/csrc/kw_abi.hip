@@ -1157,6 +1157,21 @@ int kw_fno_head_gather_bf16(int64_t B, int C, int K, const void* v, const int* c
   return KW_OK;
 }
 
+int kw_fno_lift_bf16(int64_t B, int C, int N, const float* cmap, float cmid, float chalf, const float* lc, const void* g, void* v,
+                     void* stream) {
+  if (!cmap || !lc || !g || !v || B < 1 || N < 4) return fail(KW_EINVAL, "FNO lift: NULL pointer or empty shape");
+  if (C != 16 && C != 32) return fail(KW_EUNSUPPORTED, "FNO lift: channel count must be 16 or 32");
+  if (N % 4) return fail(KW_EINVAL, "FNO lift: the coarse grid edge must be a multiple of 4");
+  if (!(chalf != 0.f)) return fail(KW_EINVAL, "FNO lift: the normalising half-range must be nonzero");
+  if (B * N * (N / 4) > (int64_t)0xffffffff * 256 / 2) return fail(KW_EINVAL, "FNO lift: too many samples");
+  if (((uintptr_t)cmap & 15) || (((uintptr_t)g | (uintptr_t)v) & 7))
+    return fail(KW_EINVAL, "FNO lift: the map must be 16-byte and the bf16 arrays 8-byte aligned");
+  hipError_t e = launch_fno_lift(B, C, N, cmap, cmid, chalf, lc, static_cast<const uint16_t*>(g), static_cast<uint16_t*>(v),
+                                 static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(KW_EHIP, std::string("FNO lift launch: ") + hipGetErrorString(e));
+  return KW_OK;
+}
+
 int kw_fno_dft_bf16(int dir, int64_t S, const void* in, const void* m1, const void* m2, void* out, void* stream) {
   if (!in || !m1 || !m2 || !out || S < 1) return fail(KW_EINVAL, "FNO DFT: NULL pointer or empty batch");
   if (dir != 1 && dir != -1) return fail(KW_EINVAL, "FNO DFT: direction must be 1 (forward) or -1 (inverse)");

@@ -383,6 +383,59 @@ __global__ __launch_bounds__(256) void k_dft_inv(int64_t S, const float* __restr
   }
 }
 
+// Input lift of the surrogate for inference, one pass from the fine map to the
+// first layer's bf16 channels: with x the 2 x 2 average of the normalised fine
+// map (cmap - cmid) / chalf,
+//   v[b][c][i][j] = bf16(x[b][i][j] lc[c] + g[c][i][j])
+// (g the lift of the coordinate grid, bf16).  A thread owns four consecutive j of
+// one row i of one sample: two 32-byte reads of the fine map, then per channel
+// one 8-byte read of g (cached) and one 8-byte write.
+template <int C>
+__global__ __launch_bounds__(256) void k_fno_lift(int64_t B, int N, const float* __restrict__ cmap, float cmid, float chalf,
+                                                  const float* __restrict__ lc, const uint16_t* __restrict__ g,
+                                                  uint16_t* __restrict__ v) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = N / 4;
+  if (t >= B * N * q) return;
+  const int64_t b = t / ((int64_t)N * q);
+  const int i = (int)((t / q) % N), j0 = (int)(t % q) * 4;
+  const float* r0 = cmap + (b * 2 * N + 2 * i) * 2 * N + 2 * j0;
+  const float* r1 = r0 + 2 * N;
+  const float4 a0 = *reinterpret_cast<const float4*>(r0), a1 = *reinterpret_cast<const float4*>(r0 + 4);
+  const float4 c0 = *reinterpret_cast<const float4*>(r1), c1 = *reinterpret_cast<const float4*>(r1 + 4);
+  const float top[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+  const float bot[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+  float x[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float s0 = (top[2 * k] - cmid) / chalf + (top[2 * k + 1] - cmid) / chalf;
+    const float s1 = (bot[2 * k] - cmid) / chalf + (bot[2 * k + 1] - cmid) / chalf;
+    x[k] = 0.25f * (s0 + s1);
+  }
+  const int64_t pos = (int64_t)i * N + j0;
+#pragma unroll 8
+  for (int c = 0; c < C; ++c) {
+    const ushort4 gq = *reinterpret_cast<const ushort4*>(g + (int64_t)c * N * N + pos);
+    const float l = lc[c];
+    const unsigned short o0 = (unsigned short)bf16_rne(fmaf(x[0], l, __uint_as_float((unsigned)gq.x << 16)));
+    const unsigned short o1 = (unsigned short)bf16_rne(fmaf(x[1], l, __uint_as_float((unsigned)gq.y << 16)));
+    const unsigned short o2 = (unsigned short)bf16_rne(fmaf(x[2], l, __uint_as_float((unsigned)gq.z << 16)));
+    const unsigned short o3 = (unsigned short)bf16_rne(fmaf(x[3], l, __uint_as_float((unsigned)gq.w << 16)));
+    *reinterpret_cast<ushort4*>(v + (b * C + c) * N * N + pos) = make_ushort4(o0, o1, o2, o3);
+  }
+}
+
+hipError_t launch_fno_lift(int64_t B, int C, int N, const float* cmap, float cmid, float chalf, const float* lc, const uint16_t* g,
+                           uint16_t* v, hipStream_t st) {
+  const int64_t threads = B * N * (N / 4);
+  const dim3 grid((unsigned)((threads + 255) / 256));
+  if (C == 32)
+    hipLaunchKernelGGL(k_fno_lift<32>, grid, dim3(256), 0, st, B, N, cmap, cmid, chalf, lc, g, v);
+  else
+    hipLaunchKernelGGL(k_fno_lift<16>, grid, dim3(256), 0, st, B, N, cmap, cmid, chalf, lc, g, v);
+  return hipGetLastError();
+}
+
 // Sensor-head gather of the surrogate: sel[b][k][c][x] = v[b][c][x][cols[k]] for
 // k < K (bf16, v [B][C][64][64]).  One workgroup per (b, c) slab: the slab is
 // staged in LDS with a padded row pitch (the column reads of different rows

@@ -38,6 +38,8 @@ hipError_t launch_setint(int* p, int n, int v, hipStream_t st);
 // surrogate (kw_fno.hip)
 hipError_t launch_mode_gemm(int M, int R, int N, int K, const uint16_t* A, const uint16_t* Bt, float* C, hipStream_t st);
 hipError_t launch_head_gather(int64_t B, int C, int K, const uint16_t* v, const int* cols, uint16_t* sel, hipStream_t st);
+hipError_t launch_fno_lift(int64_t B, int C, int N, const float* cmap, float cmid, float chalf, const float* lc, const uint16_t* g,
+                           uint16_t* v, hipStream_t st);
 hipError_t launch_dft_fwd(int64_t S, const uint16_t* v, const uint16_t* mx, const uint16_t* myt, float* x, hipStream_t st);
 hipError_t launch_dft_inv(int64_t S, const float* y, const uint16_t* nyt, const uint16_t* nx, uint16_t* out, hipStream_t st);
 hipError_t launch_fno_pointwise(int B, int C, int64_t P, const uint16_t* v, const uint16_t* s, const float* pw, const float* pb,

@@ -26,7 +26,7 @@ EXPORTS = (
     "kw_plan_set_sensor", "kw_plan_init", "kw_plan_run", "kw_plan_read", "kw_plan_field_size",
     "kw_plan_destroy", "kw_spectral_apply", "kw_plan_run_profiled", "kw_plan_launch", "kw_plan_sync", "kw_kernel_kind_count", "kw_kernel_kind_name",
     "kw_plan_adjoint_c", "kw_mode_gemm_bf16", "kw_mode_contract_bf16", "kw_plan_tiles", "kw_plan_radix_plans", "kw_plan_set_density_split", "kw_fno_pointwise_bf16",
-    "kw_fno_dft_bf16", "kw_fno_head_gather_bf16", "kw_spectral_apply_static", "kw_batch_create", "kw_batch_run",
+    "kw_fno_dft_bf16", "kw_fno_head_gather_bf16", "kw_fno_lift_bf16", "kw_spectral_apply_static", "kw_batch_create", "kw_batch_run",
     "kw_batch_destroy",
 )
 
@@ -107,6 +107,7 @@ def lib():
         "kw_fno_pointwise_bf16": ([i32, i32, i64, vp, vp, vp, vp, vp, i32, vp], i32),
         "kw_fno_dft_bf16": ([i32, i64, vp, vp, vp, vp, vp], i32),
         "kw_fno_head_gather_bf16": ([i64, i32, i32, vp, vp, vp, vp], i32),
+        "kw_fno_lift_bf16": ([i64, i32, i32, vp, C.c_float, C.c_float, vp, vp, vp, vp], i32),
         "kw_kernel_kind_count": ([], i32),
         "kw_kernel_kind_name": ([i32], C.c_char_p),
     }

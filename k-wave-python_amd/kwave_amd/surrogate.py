@@ -119,6 +119,25 @@ def mode_contract(x, bt, md):
     return y
 
 
+def fno_lift(cmap, lc, g):
+    """First-layer channels of the inference forward in one pass (kw_fno_lift_bf16): bf16
+    (B, C, n, n) = (2 x 2 average of (cmap - C_MID) / C_HALF) * lc[c] + g[c], from the fine
+    maps cmap (B, 2n, 2n) float32, lc (C,) float32 and the grid lift g (C, n, n) bf16."""
+    import torch
+
+    cmap = cmap.float().contiguous()
+    b, n2 = cmap.shape[0], cmap.shape[-1]
+    n = n2 // 2
+    lc = lc.detach().float().reshape(-1).contiguous()
+    g = g.detach().to(torch.bfloat16).reshape(-1, n, n).contiguous()
+    c = lc.numel()
+    v = torch.empty((b, c, n, n), dtype=torch.bfloat16, device=cmap.device)
+    stream = torch.cuda.current_stream(cmap.device).cuda_stream
+    _abi.check(_abi.lib().kw_fno_lift_bf16(b, c, n, C.c_void_p(cmap.data_ptr()), C_MID, C_HALF, C.c_void_p(lc.data_ptr()),
+                                           C.c_void_p(g.data_ptr()), C.c_void_p(v.data_ptr()), C.c_void_p(stream)))
+    return v
+
+
 def fno_pointwise(v, s, pw, pb, gelu):
     """act(s + pb + pw v) per sample and position in one pass (inference; kw_fno_pointwise_bf16).
 
@@ -421,14 +440,16 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
             normalised map, and the prediction's distance from the mean trace set."""
             b = cmap.shape[0]
             n = N_COARSE
-            cn = (cmap - C_MID) / C_HALF
-            x = F.avg_pool2d(cn[:, None], 2)                                     # (b, 1, n, n)
-            v = x * self.lift_c[None, :, None, None] + self.lift_g(self.grid)    # lift of (c, x, y)
             wd = self.width
-            # inference under bf16 autocast: each layer's pointwise part is one fused kernel
-            fused = not torch.is_grad_enabled() and v.is_cuda and torch.is_autocast_enabled()
+            # inference under bf16 autocast: the lift and each layer's pointwise part are one
+            # fused kernel each
+            fused = not torch.is_grad_enabled() and cmap.is_cuda and torch.is_autocast_enabled()
             if fused:
-                v = v.to(torch.bfloat16)
+                v = fno_lift(cmap, self.lift_c, self.lift_g(self.grid))
+            else:
+                cn = (cmap - C_MID) / C_HALF
+                x = F.avg_pool2d(cn[:, None], 2)                                     # (b, 1, n, n)
+                v = x * self.lift_c[None, :, None, None] + self.lift_g(self.grid)    # lift of (c, x, y)
             for i in range(self.layers):
                 if fused:
                     v = fno_pointwise(v, self.spec[i](v), self.pw[i], self.pb[i], i < self.layers - 1)
@@ -444,13 +465,13 @@ def build_model(device="cuda", seed=0, width=WIDTH, layers=LAYERS, modes=MODES):
                 sel = v.permute(0, 3, 1, 2).index_select(1, self.cols_ab).reshape(b * self.n_sens, 2 * wd * n)
             wab = torch.cat([self.head_a.weight, self.head_b.weight], 1)
             ha = F.linear(sel, wab, self.head_a.bias).reshape(b, self.n_sens, -1)
-            ff = cn.index_select(2, self.cols_f).permute(0, 2, 1)                # (b, n_sens, 128)
+            ff = ((cmap.index_select(2, self.cols_f) - C_MID) / C_HALF).permute(0, 2, 1)  # (b, n_sens, 128)
             out = self.mean[None] + ha + self.head_f(ff)
             if not features:
                 return out
             vf = v.float()
             dev = (out.float() - self.mean[None]).flatten(1).norm(dim=1) / self.mean.norm()
-            flat = cn.flatten(1)
+            flat = ((cmap - C_MID) / C_HALF).flatten(1)
             feats = torch.cat([vf.mean((2, 3)), vf.std((2, 3)), flat.std(1, keepdim=True),
                                flat.abs().amax(1, keepdim=True), dev[:, None]], 1)
             return out, feats

@@ -219,6 +219,27 @@ def test_fused_pointwise_matches_fp32_reference(c, p, gelu):
     assert float(err.mean()) < 2.0 ** -8 * float(ref.abs().mean())
 
 
+@pytest.mark.parametrize("c", [16, 32])
+def test_fused_lift_matches_fp32_reference(c):
+    """kw_fno_lift_bf16 == bf16 of (2 x 2 mean of the normalised map) * lc + g in FP32, to one
+    bf16 rounding (the kernel fuses the multiply-add)."""
+    import torch
+    import torch.nn.functional as F
+
+    from kwave_amd import surrogate
+
+    torch.manual_seed(5)
+    b, n = 37, 64
+    cmap = surrogate.C_MID + 120.0 * torch.rand(b, 2 * n, 2 * n, device="cuda") - 60.0
+    lc = torch.randn(c, device="cuda")
+    g = torch.randn(c, n, n, device="cuda").to(torch.bfloat16)
+    got = surrogate.fno_lift(cmap, lc, g).float()
+    x = F.avg_pool2d(((cmap.double() - surrogate.C_MID) / surrogate.C_HALF)[:, None], 2)
+    ref = x * lc.double()[None, :, None, None] + g.double()[None]
+    err = (got.double() - ref).abs() / ref.abs().clamp_min(1e-3)
+    assert float(err.max()) <= 1.01 * 2.0 ** -8, float(err.max())
+
+
 def test_fused_inference_matches_unfused_forward():
     """The fused inference forward of the surrogate network agrees with its unfused
     (autograd) form under the same bf16 autocast to within bf16 rounding."""
