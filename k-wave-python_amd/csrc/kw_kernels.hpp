@@ -229,9 +229,6 @@ __device__ __forceinline__ void set_lane(float2 (&a)[E], int m, int h, float v) 
 #ifndef KW_WPE_COL
 #define KW_WPE_COL 1
 #endif
-#ifndef KW_COL_NOPRE
-#define KW_COL_NOPRE 0
-#endif
 #ifndef KW_WPE_COLOPS
 #define KW_WPE_COLOPS 4  // the prefetch of the next job input would take it to 130 VGPRs, 3 waves
 #endif
@@ -759,27 +756,7 @@ __global__ __launch_bounds__((ColCfg<N, L, EM, PV>::NT)) KW_WPE(N, (OPS ? KW_WPE
   const float2* tw = twl;
 
   float2 v[C::E];
-  if constexpr (!OPS && KW_COL_NOPRE) {
-    // one buffer of one tile at a time: load, transform, store (latency hidden by
-    // the other workgroups of the CU)
-    const int ntiles = a.nouter * ntile;
-    for (int lt = lid; lt < ntiles; lt += (int)gridDim.x) {
-      const int ot = lt / ntile, colt = (lt % ntile) * L + c;
-      const bool vt = colt < a.ncols;
-      const int64_t bt = (int64_t)b * g.nspec + (int64_t)ot * a.ostride + colt;
-      for (int q = 0; q < a.nbuf; ++q) {
-        const auto bp = gw(fl.spec[a.bufs[q]]);
-#pragma unroll
-        for (int m = 0; m < C::E; ++m) v[m] = vt ? ld2(bp, bt + (int64_t)(j0 + m * C::TPL) * a.estride) : make_float2(0.f, 0.f);
-        if (a.mode == 0) fft_reg<N, -1, typename C::Idx, EM, PV>(v, lds, tw, c, j0, idx);
-        else fft_reg<N, +1, typename C::Idx, EM, PV>(v, lds, tw, c, j0, idx);
-        if (vt) {
-#pragma unroll
-          for (int m = 0; m < C::E; ++m) st2(bp, bt + (int64_t)(j0 + m * C::TPL) * a.estride, v[m]);
-        }
-      }
-    }
-  } else if constexpr (!OPS) {
+  if constexpr (!OPS) {
     // software pipeline over the buffers and tiles: the loads of the next buffer
     // (after a tile's last buffer, the first buffer of this workgroup's next tile)
     // are in flight while the current one is transformed and stored
