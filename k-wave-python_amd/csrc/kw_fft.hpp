@@ -50,12 +50,14 @@ struct RadixPlan {
 // Plan variant pv = 1 (lengths with a factor 5 or 7 only): radix 4 next to 5 or 7
 // (E = 20, 28) - fewer stages, and so fewer LDS exchanges, for twice the
 // registers; the column passes time both variants per plan and keep the faster.
+// Plan variant pv = 2 (powers of two only): radix 4 then 2 (E = 4) - twice the
+// exchanges of the radix-8 plan for half the registers per thread.
 constexpr RadixPlan make_plan(int n, int pv = 0) {
   RadixPlan p{n, 0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
   int odd = n;
   while (odd > 1 && odd % 2 == 0) odd /= 2;
   const bool f57 = odd % 5 == 0 || odd % 7 == 0;
-  const int p2max = odd == 1 ? 8 : f57 ? (pv == 1 ? 4 : 2) : 4;
+  const int p2max = odd == 1 ? (pv == 2 ? 4 : 8) : f57 ? (pv == 1 ? 4 : 2) : 4;
   int m = n;
   while (p2max >= 8 && m % 8 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 8; m /= 8; }
   while (p2max >= 4 && m % 4 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 4; m /= 4; }

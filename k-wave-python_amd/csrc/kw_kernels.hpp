@@ -28,9 +28,15 @@ constexpr int pow2floor(int v) {
 }
 constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+#ifndef KW_ZPV
+#define KW_ZPV 0
+#endif
 template <int N>
 struct ZCfg {
-  using F = FftShape<N>;
+  // radix plan of the row passes: KW_ZPV=2 selects the radix-4 plan (4 elements per
+  // thread) for power-of-two lengths
+  static constexpr int PV = (KW_ZPV == 2 && (N & (N - 1)) == 0) ? 2 : 0;
+  using F = FftShape<N, 1, PV>;
   static constexpr int E = F::E;
   static constexpr int TPL = F::TPL;
   static constexpr int L0 = clampi(pow2floor(2048 / N), 1, 64);
@@ -143,7 +149,7 @@ __device__ __forceinline__ void inv_to_regs(float2* lds, const float2* __restric
 #pragma unroll
   for (int m = 0; m < C::E; ++m) out[m] = lds[idx(j0 + m * C::TPL, c)];
   __syncthreads();
-  fft_reg<N, +1>(out, lds, tw, c, j0, idx);
+  fft_reg<N, +1, typename C::Idx, 1, C::PV>(out, lds, tw, c, j0, idx);
 }
 
 // real values of a row pair at the owned positions -> half spectra of both rows
@@ -153,7 +159,7 @@ __device__ __forceinline__ void fwd_from_regs(float2* lds, float2 (&v)[ZCfg<N>::
   using C = ZCfg<N>;
   const typename C::Idx idx{};
   const int j0 = tid % C::TPL, c = tid / C::TPL;
-  fft_reg<N, -1>(v, lds, tw, c, j0, idx);
+  fft_reg<N, -1, typename C::Idx, 1, C::PV>(v, lds, tw, c, j0, idx);
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < C::E; ++m) lds[idx(j0 + m * C::TPL, c)] = v[m];
@@ -250,9 +256,9 @@ __device__ __forceinline__ void fill_stage_tw(float2* dst, const float2* src, in
 #define KW_ZPASS_PROLOGUE                                              \
   using C = ZCfg<N>;                                                   \
   __shared__ float2 lds[C::LDS];                                       \
-  __shared__ float2 twl[stage_tw_size<N>()];                           \
+  __shared__ float2 twl[stage_tw_size<N, C::PV>()];                    \
   const int tid = threadIdx.x;                                         \
-  fill_stage_tw<N, 0>(twl, tw_g, tid, C::NT);                          \
+  fill_stage_tw<N, C::PV>(twl, tw_g, tid, C::NT);                      \
   __syncthreads();                                                     \
   const float2* tw = twl;                                              \
   const int b = blockIdx.y;                                            \

@@ -113,6 +113,7 @@ template <int N>
 void lengths() {
   lengths_pv<N, 0>();
   if constexpr (has_plan_variant(N)) lengths_pv<N, 1>();
+  if constexpr ((N & (N - 1)) == 0) lengths_pv<N, 2>();  // radix-4 plan of the row passes
 }
 
 int main() {
