@@ -190,13 +190,16 @@ int kw_plan_destroy(kw_plan* plan);
  * one segment at a time).  adj = dJ/d(recorded p), [batch][nrec][nsens] (the
  * device record layout); grad_c: [batch][grid size] out.  The gradient holds
  * c_ref, dt, the PML and kappa fixed and includes the dependence of the
- * additive source scale 2 dt / (D c0 d) on c0 at the source points.  Linear
- * lossless plans with additive (or uncorrected additive) pressure sources, no
- * p0 and no velocity sources; every instance records p with a common Nt,
- * sensor count and record start.  The plan's fields and records afterwards are
- * those of the forward run's first segment replay: call kw_plan_init before
- * stepping it again.  The reference has no adjoint: this differentiates its
- * Simulation.step (kspace_solver.py:691-747) and source scaling (:619-624). */
+ * additive source scale 2 dt / (D c0 d) on c0 at the source points and, with an
+ * initial pressure p0, of the initial split densities p0 / (c0^2 p0div_i) (the
+ * state after step 0 is set from p0, kspace_solver.py:729-733, and only those
+ * densities depend on c0).  Linear lossless plans with additive (or uncorrected
+ * additive) pressure sources and/or p0, no velocity sources; every instance
+ * records p with a common Nt, sensor count and record start.  The plan's fields
+ * and records afterwards are those of the forward run's first segment replay:
+ * call kw_plan_init before stepping it again.  The reference has no adjoint:
+ * this differentiates its Simulation.step (kspace_solver.py:691-747) and source
+ * scaling (:619-624). */
 int kw_plan_adjoint_c(kw_plan* plan, const float* adj, int ckpt, float* grad_c);
 
 /* Mode-batched GEMM of the Fourier-neural-operator surrogate, BF16 inputs with

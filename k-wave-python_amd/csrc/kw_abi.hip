@@ -1052,7 +1052,6 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   for (int b = 0; b < nb; ++b) {
     if (!P->inst_set[b]) return fail(KW_EINVAL, "instance " + std::to_string(b) + " not set");
     const kw::Inst& I = P->h_inst[b];
-    if (P->p0[b]) return fail(KW_EUNSUPPORTED, "adjoint: initial pressure is not differentiated");
     for (int i = 0; i < D; ++i)
       if (I.srcu[i].mode != SRC_NONE) return fail(KW_EUNSUPPORTED, "adjoint: velocity sources are not differentiated");
     if (I.srcp.mode == SRC_DIRICHLET) return fail(KW_EUNSUPPORTED, "adjoint: dirichlet pressure sources are not differentiated");
@@ -1066,7 +1065,11 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   if ((rc = sync_inst(P))) return rc;
   const int nt = I0.nt;
   const int nrec = nt - I0.rec_start;
-  const int nck = (nt + ckpt - 1) / ckpt;
+  // with an initial pressure the state after step 0 is set by kw_plan_init (p = p0,
+  // rho_i = p0 / (c0^2 p0div_i), u_i from grad p0), so the stepped range starts at 1
+  const bool with_p0 = P->p0[0] != nullptr;
+  const int tb = with_p0 ? 1 : 0;
+  const int nck = (nt - tb + ckpt - 1) / ckpt;
   const int64_t nf = (int64_t)nb * g.ntot;
   const int nstate = 2 * D + 1;  // p, u_i, rho_i
   DevBuf tmp;                    // freed when the call returns
@@ -1095,8 +1098,8 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   if ((rc = kw_plan_init(P))) return rc;
   P->t_host = -1;  // the sweep below sets the step counter itself
   P->skip_gather = false;
-  for (int t = 0; t < nt; ++t) {
-    if (t % ckpt == 0 && (rc = copy_state(t / ckpt, true))) return rc;
+  for (int t = tb; t < nt; ++t) {
+    if ((t - tb) % ckpt == 0 && (rc = copy_state((t - tb) / ckpt, true))) return rc;
     if ((rc = enqueue_step(P))) return rc;
   }
   // reverse sweep, one checkpoint segment at a time
@@ -1108,7 +1111,7 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   int* tdev = P->fl.tdev;
   P->skip_gather = true;
   for (int seg = nck - 1; seg >= 0 && rc == KW_OK; --seg) {
-    const int t0 = seg * ckpt, t1 = std::min(nt, t0 + ckpt);
+    const int t0 = tb + seg * ckpt, t1 = std::min(nt, t0 + ckpt);
     P->fl.tdev = tdev;
     if ((rc = copy_state(seg, false))) break;
     // a step starts from F_z[p], which the previous step's last kernel leaves in
@@ -1132,6 +1135,18 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   P->fl.tdev = tdev;
   P->skip_gather = false;
   if (rc) { (void)hipStreamSynchronize(P->st); return rc; }
+  if (with_p0) {  // the c-dependence of the set state: rho_i = p0 / (c0^2 p0div_i)
+    for (int b = 0; b < nb && rc == KW_OK; ++b) {
+      const kw::Inst& I = P->h_inst[b];
+      const int64_t off = (int64_t)b * g.ntot;
+      const float* rb[3] = {af.rb[0] + off, D > 1 ? af.rb[1] + off : nullptr, D > 2 ? af.rb[2] + off : nullptr};
+      float w[3] = {0.f, 0.f, 0.f};
+      for (int i = 0; i < D; ++i) w[i] = std::isinf(I.p0div[i]) ? 0.f : 1.f / I.p0div[i];
+      if (launch_adj_p0(g.ntot, D, P->p0[b], I.c0sq, I.c0sq_s, rb, w, af.cb + off, P->st) != hipSuccess)
+        rc = fail(KW_EHIP, "adjoint: initial-pressure term launch failed");
+    }
+    if (rc) { (void)hipStreamSynchronize(P->st); return rc; }
+  }
   KW_HIP(hipMemcpyAsync(grad_c, af.cb, sizeof(float) * nf, hipMemcpyDeviceToHost, P->st));
   KW_HIP(hipStreamSynchronize(P->st));
   return KW_OK;

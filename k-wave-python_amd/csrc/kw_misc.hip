@@ -1,4 +1,6 @@
 // Launchers instantiating the time-loop kernels (see kw_kernels.hpp).
+#include <algorithm>
+
 #include "kw_kernels.hpp"
 
 namespace kw {
@@ -96,6 +98,29 @@ __global__ void k_adj_srcgrad(Geom g, AdjFields af, const float* __restrict__ Zk
       cb[idx] += -(s.scale[e] / c) * sv * Z[idx];
     }
   }
+}
+
+// Initial-pressure term of dJ/dc0 (one instance): the state after step 0 holds
+// rho_i = p0 / (c0^2 p0div_i), so cb += -2 p0 / c0^3 sum_i rb_i / p0div_i with rb_i
+// the adjoints of that state (p0div_i infinite for an axis that gets no share).
+__global__ void k_adj_p0(int64_t n, int D, const float* __restrict__ p0, const float* __restrict__ c0sq, int c0sq_s,
+                         const float* __restrict__ rb0, const float* __restrict__ rb1, const float* __restrict__ rb2, float w0,
+                         float w1, float w2, float* __restrict__ cb) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float r = w0 * rb0[i];
+    if (D > 1) r += w1 * rb1[i];
+    if (D > 2) r += w2 * rb2[i];
+    const float c2 = c0sq[i * c0sq_s];
+    cb[i] += -2.f * p0[i] / (c2 * sqrtf(c2)) * r;
+  }
+}
+
+hipError_t launch_adj_p0(int64_t n, int D, const float* p0, const float* c0sq, int c0sq_s, const float* const rb[3], const float w[3],
+                         float* cb, hipStream_t st) {
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_adj_p0, dim3((unsigned)blocks), dim3(256), 0, st, n, D, p0, c0sq, c0sq_s, rb[0], D > 1 ? rb[1] : rb[0],
+                     D > 2 ? rb[2] : rb[0], w[0], w[1], w[2], cb);
+  return hipGetLastError();
 }
 
 hipError_t launch_adj_scatter(const Geom& g, int nb, int max_sens, const AdjFields& af, const Inst* inst, int t, hipStream_t st) {

@@ -27,6 +27,14 @@ Dop_i^T = -Gop_i and K^T = K.  For J with dJ/dy_t = r_t the reverse sweep is
 c_ref, dt, the PML and kappa are held fixed (they depend on c only through
 c_ref = max(c0), which the gradient treats as frozen).
 
+With an initial pressure p0 the reference overrides the state after step 0
+(kspace_solver.py:729-733): p^0 = p0, rho_i^0 = p0 / (D c^2) and
+u_i^0 = (dt / (2 rho0_sg_i)) Gop_i p0, so nothing of step 0 itself reaches the
+traces.  Of that state only rho_i^0 depends on c, which gives the last term
+    cbar += -2 p0 / (D c^3) sum_i rbar_i
+with rbar_i the adjoint of rho_i^0 (what the reverse sweep of step 1 leaves);
+the record of p^0 = p0 and the step-0 source term carry no c-dependence.
+
 Pins: finite differences of OracleSimulation in FP64 (tests/test_oracle_adjoint.py).
 """
 from __future__ import annotations
@@ -65,8 +73,6 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
         raise ValueError("adjoint: pass a fresh simulation")
     if sim.tau is not None or sim.eta is not None or sim.BonA is not None:
         raise ValueError("adjoint: linear lossless media only")
-    if sim.p0 is not None:
-        raise ValueError("adjoint: initial pressure is not differentiated")
     if sim.cart is not None or sim.sensor_idx is None:
         raise ValueError("adjoint: binary sensor masks only")
     if any(_is_enabled(getattr(sim.source, n, 0)) for n in ("ux", "uy", "uz")):
@@ -87,7 +93,8 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
     ubar = [np.zeros(shape) for _ in range(nd)]
     rbar = [np.zeros(shape) for _ in range(nd)]
     cbar = np.zeros(shape)
-    for t in range(sim.Nt - 1, -1, -1):
+    t_end = 1 if sim.p0 is not None else 0  # with p0 the state after step 0 is set, not stepped
+    for t in range(sim.Nt - 1, t_end - 1, -1):
         if t >= sim.start:
             flat = pbar.ravel()
             flat[sim.sensor_idx] += adj_src[:, t - sim.start]
@@ -112,4 +119,11 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
             ubar[i] = sim.pml_sg[i] ** 2 * ubar[i]
             pnew += sim._diff(v, sim.op_div[i])
         pbar = pnew
+    if sim.p0 is not None:
+        cbar += _initial_density_term(sim, c0, rbar)
     return cbar, rec
+
+
+def _initial_density_term(sim, c0, rbar):
+    """d/dc of rho_i^0 = p0 / (D c^2) against the adjoints rbar_i of rho_i^0 (kspace_solver.py:732)."""
+    return -2.0 * sim.p0 / (sim.ndim * c0 ** 3) * sum(rbar)

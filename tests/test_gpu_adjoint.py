@@ -5,7 +5,8 @@ adjoint source is dJ/dy = y - d.  Checks:
 
 * the GPU gradient equals the FP64 CPU adjoint (oracle/kspace_adjoint.py, itself
   pinned by finite differences in tests/test_oracle_adjoint.py) to FP32 accuracy,
-  in 2-D with both additive source modes and in 3-D;
+  in 2-D with both additive source modes and in 3-D, with an initial pressure p0
+  beside the line source and with p0 alone;
 * central finite differences of the GPU solver at 20 seeded cells (seed 7) of the
   128^2 surrogate problem, 600 steps, in the region the wave reaches and returns
   from.  h = 5 m/s: the FP32 solver's rounding noise in J (~1e4 of J ~ 1e13 over
@@ -56,9 +57,12 @@ def _misfit_data(y, seed=5):
     return 0.7 * y + 0.05 * np.abs(y).max() * np.random.default_rng(seed).standard_normal(y.shape)
 
 
-@pytest.mark.parametrize("shape,mode", [((32, 40), "additive"), ((32, 40), "additive-no-correction"), ((16, 24, 32), "additive")])
-def test_gpu_adjoint_matches_fp64_adjoint(shape, mode):
-    case = adjoint_case(shape=shape, nt=90 if len(shape) == 2 else 60, mode=mode)
+@pytest.mark.parametrize("shape,mode,p0,line", [((32, 40), "additive", False, True), ((32, 40), "additive-no-correction", False, True),
+                                                ((16, 24, 32), "additive", False, True), ((32, 40), "additive", True, True),
+                                                ((32, 40), None, True, False), ((16, 24, 32), None, True, False)])
+def test_gpu_adjoint_matches_fp64_adjoint(shape, mode, p0, line):
+    """Line source, initial pressure p0 beside it, or p0 alone (the initial-density term of dJ/dc0)."""
+    case = adjoint_case(shape=shape, nt=90 if len(shape) == 2 else 60, mode=mode or "additive", p0=p0, line_source=line)
     y = _gpu_traces(case)
     d = _misfit_data(y)
     g_gpu, y2 = _gpu_grad(case, y - d, checkpoint=16)
@@ -66,7 +70,7 @@ def test_gpu_adjoint_matches_fp64_adjoint(shape, mode):
     g_ref, y_ref = _oracle_grad(case, y - d)
     assert _rel(y, y_ref) < 1e-4
     err = _rel(g_gpu, g_ref)
-    print(f"{shape} {mode}: gradient rel-L2 vs FP64 adjoint {err:.2e}")
+    print(f"{shape} {mode} p0={p0}: gradient rel-L2 vs FP64 adjoint {err:.2e}")
     assert err < 1e-3, err
 
 

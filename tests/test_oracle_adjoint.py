@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 
-def adjoint_case(shape=(32, 40), nt=90, seed=7, mode="additive"):
+def adjoint_case(shape=(32, 40), nt=90, seed=7, mode="additive", p0=False, line_source=True):
     from kwave_amd.data import Vector
     from kwave_amd.kgrid import kWaveGrid
     from kwave_amd.kmedium import kWaveMedium
@@ -29,11 +29,16 @@ def adjoint_case(shape=(32, 40), nt=90, seed=7, mode="additive"):
     rho = 1000.0 + 100.0 * gaussian_filter(rng.random(shape), 2)
     med = kWaveMedium(sound_speed=c, density=rho)
     src = kSource()
-    mask = np.zeros(shape, dtype=bool)
-    mask[(8,) + tuple(slice(s // 2 - 3, s // 2 + 3) for s in shape[1:])] = True
-    src.p_mask = mask
-    src.p = 1e4 * tone_burst(1.0 / kg.dt, 2.5e6, 2)
-    src.p_mode = mode
+    if line_source:
+        mask = np.zeros(shape, dtype=bool)
+        mask[(8,) + tuple(slice(s // 2 - 3, s // 2 + 3) for s in shape[1:])] = True
+        src.p_mask = mask
+        src.p = 1e4 * tone_burst(1.0 / kg.dt, 2.5e6, 2)
+        src.p_mode = mode
+    if p0:  # an initial pressure blob beside the line source
+        grid = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+        r2 = sum((g - s // 3) ** 2 for g, s in zip(grid, shape))
+        src.p0 = 2e3 * np.exp(-r2 / 8.0)
     smask = np.zeros(shape, dtype=bool)
     smask[(shape[0] - 9,) + tuple(slice(s // 4, 3 * s // 4, 2) for s in shape[1:])] = True
     sen = kSensor(smask)
@@ -60,11 +65,11 @@ def _cells(case, c, n_rand=6, seed=3):
     return cells
 
 
-@pytest.mark.parametrize("mode", ["additive", "additive-no-correction"])
-def test_oracle_adjoint_matches_finite_differences(mode):
+@pytest.mark.parametrize("mode,p0", [("additive", False), ("additive-no-correction", False), ("additive", True)])
+def test_oracle_adjoint_matches_finite_differences(mode, p0):
     from oracle.kspace_adjoint import sound_speed_gradient
 
-    case = adjoint_case(mode=mode)
+    case = adjoint_case(mode=mode, p0=p0)
     c = np.asarray(case[1].sound_speed, dtype=np.float64)
     rec0 = _oracle(case, c).run()["p"]
     d = rec0 * 0.7 + 0.05 * np.abs(rec0).max() * np.random.default_rng(5).standard_normal(rec0.shape)
@@ -106,3 +111,32 @@ def test_oracle_adjoint_planted_sign_error_is_caught():
     J = lambda cc: 0.5 * float(np.sum((_oracle(case, cc).run()["p"] - d) ** 2))  # noqa: E731
     fd = (J(cp) - J(cm)) / (2 * h)
     assert abs(grad.ravel()[cell] - fd) > 1e-3 * abs(fd)
+
+
+def test_oracle_adjoint_planted_initial_density_slip_is_caught():
+    """Dropping the initial-density term -2 p0 / (D c^3) sum rbar, or halving it, fails the
+    finite-difference check at a cell inside the initial pressure blob."""
+    from oracle import kspace_adjoint
+
+    case = adjoint_case(p0=True)
+    c = np.asarray(case[1].sound_speed, dtype=np.float64)
+    rec0 = _oracle(case, c).run()["p"]
+    d = 0.5 * rec0
+    J = lambda cc: 0.5 * float(np.sum((_oracle(case, cc).run()["p"] - d) ** 2))  # noqa: E731
+    p0 = np.asarray(case[2].p0)
+    cell = int(np.argmax(p0.ravel() * (c.ravel() < c.max() - 1.0)))
+    h = 1e-2
+    cp, cm = c.copy(), c.copy()
+    cp.ravel()[cell] += h
+    cm.ravel()[cell] -= h
+    fd = (J(cp) - J(cm)) / (2 * h)
+    good, _ = kspace_adjoint.sound_speed_gradient(_oracle(case, c), rec0 - d)
+    assert abs(good.ravel()[cell] - fd) <= 1e-6 * abs(fd)
+    orig = kspace_adjoint._initial_density_term
+    for planted in (lambda sim, c0, rbar: 0.0 * c0, lambda sim, c0, rbar: 0.5 * orig(sim, c0, rbar)):
+        try:
+            kspace_adjoint._initial_density_term = planted
+            bad, _ = kspace_adjoint.sound_speed_gradient(_oracle(case, c), rec0 - d)
+        finally:
+            kspace_adjoint._initial_density_term = orig
+        assert abs(bad.ravel()[cell] - fd) > 1e-3 * abs(fd)
