@@ -35,6 +35,15 @@ traces.  Of that state only rho_i^0 depends on c, which gives the last term
 with rbar_i the adjoint of rho_i^0 (what the reverse sweep of step 1 leaves);
 the record of p^0 = p0 and the step-0 source term carry no c-dependence.
 
+Absorbing media (kspace_solver.py:471-516, linear: no B/A) add to the equation of
+state c^2 (tau A - eta B) with A = L1[rho0 d] (Stokes: rho0 d), d = sum_i Dop_i u_i,
+B = L2[sum_i rho_i]; L1, L2 are real k-space multipliers (self-adjoint).  With
+w = c^2 pbar the reverse step gains
+    cbar += w ((y-1) tau A - y eta B) / c          (d tau/dc, d eta/dc at fixed A, B)
+    rbar_i += w - L2[eta w]                         (instead of += w)
+    ubar_i += Gop_i(b_i dt rho0 rbar_i - rho0 L1[tau w])   (Dop_i^T = -Gop_i)
+with A and B of each step kept from the forward sweep.
+
 Pins: finite differences of OracleSimulation in FP64 (tests/test_oracle_adjoint.py).
 """
 from __future__ import annotations
@@ -71,18 +80,22 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
     """
     if sim.t != 0:
         raise ValueError("adjoint: pass a fresh simulation")
-    if sim.tau is not None or sim.eta is not None or sim.BonA is not None:
-        raise ValueError("adjoint: linear lossless media only")
+    if sim.BonA is not None:
+        raise ValueError("adjoint: linear media only (no B/A)")
     if sim.cart is not None or sim.sensor_idx is None:
         raise ValueError("adjoint: binary sensor masks only")
     if any(_is_enabled(getattr(sim.source, n, 0)) for n in ("ux", "uy", "uz")):
         raise ValueError("adjoint: velocity sources are not differentiated")
     nd, shape = sim.ndim, sim.shape
     adj_src = np.asarray(adj_src, dtype=np.float64)
-    ps = []
+    absorbing = sim.tau is not None or sim.eta is not None
+    ps, As, Bs = [], [], []
     while sim.t < sim.Nt:
         sim.step()
         ps.append(sim.p.copy())
+        if absorbing:  # the absorption and dispersion operands of this step's equation of state
+            As.append(_absorption_operand(sim))
+            Bs.append(sim._diff(sum(sim.rho), sim.nabla2) if sim.eta is not None else None)
     rec = sim.data["p"].copy()
     src = _source_points(sim)
     c0 = sim.c0 if np.ndim(sim.c0) else np.full(shape, float(sim.c0))
@@ -99,8 +112,13 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
             flat = pbar.ravel()
             flat[sim.sensor_idx] += adj_src[:, t - sim.start]
             pbar = flat.reshape(shape)
-        rbar = [rbar[i] + c0_sq * pbar for i in range(nd)]
+        w = c0_sq * pbar
         cbar += 2 * ps[t] * pbar / c0
+        dbar = 0.0
+        if absorbing:
+            cbar += _absorption_speed_term(sim, c0, w, As[t], Bs[t])
+            dbar, w = _absorption_adjoint(sim, rho0, w)
+        rbar = [rbar[i] + w for i in range(nd)]
         if src is not None:
             pts, sig, rows, base, c, mode = src
             if t < sig.shape[1]:
@@ -110,7 +128,7 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
                 flat[pts] += -(base / c) * sig[rows, t] * Z.ravel()[pts]
                 cbar = flat.reshape(shape)
         for i in range(nd):
-            w = sim.pml[i] * sim.dt * rho0 * rbar[i]
+            w = sim.pml[i] * sim.dt * rho0 * rbar[i] - dbar
             rbar[i] = sim.pml[i] ** 2 * rbar[i]
             ubar[i] = ubar[i] + sim._diff(w, sim.op_grad[i])
         pnew = np.zeros(shape)
@@ -122,6 +140,40 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
     if sim.p0 is not None:
         cbar += _initial_density_term(sim, c0, rbar)
     return cbar, rec
+
+
+def _absorption_operand(sim):
+    """What tau multiplies in the equation of state of the step just taken: rho0 div u
+    (Stokes) or L1[rho0 div u] (power law), div u = sum_i Dop_i u_i (kspace_solver.py:712-724)."""
+    if sim.tau is None:
+        return None
+    div = sum(sim._diff(sim.u[i], sim.op_div[i]) for i in range(sim.ndim))
+    return sim.rho0 * div if sim.stokes else sim._diff(sim.rho0 * div, sim.nabla1)
+
+
+def _absorption_adjoint(sim, rho0, w):
+    """(adjoint of div u through the absorption term, what rbar_i gains) for w = c^2 pbar:
+    rho0 L1[tau w] (Stokes: rho0 tau w), and w - L2[eta w] (the dispersion term is
+    -eta L2[sum rho])."""
+    dbar = 0.0
+    if sim.tau is not None:
+        dbar = rho0 * (sim.tau * w if sim.stokes else sim._diff(sim.tau * w, sim.nabla1))
+    if sim.eta is not None:
+        w = w - sim._diff(sim.eta * w, sim.nabla2)
+    return dbar, w
+
+
+def _absorption_speed_term(sim, c0, w, A, B):
+    """c-derivative of c^2 (tau A - eta B) at fixed A, B against pbar (w = c^2 pbar):
+    tau = -2 a c^(y-1) and eta = 2 a c^y tan(pi y / 2) give d tau/dc = (y-1) tau / c and
+    d eta/dc = y eta / c (Stokes: tau = -2 a c, y = 2)."""
+    y = sim.alpha_power
+    out = 0.0
+    if sim.tau is not None:
+        out = out + w * (y - 1.0) * sim.tau / c0 * A
+    if sim.eta is not None:
+        out = out - w * y * sim.eta / c0 * B
+    return out
 
 
 def _initial_density_term(sim, c0, rbar):
