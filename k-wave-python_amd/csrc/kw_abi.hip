@@ -1004,6 +1004,15 @@ int adjoint_step(kw_plan* P, const AdjFields& af, float* Zbuf, int t) {
   const float2* twz = P->ops.tw[D - 1];
   int rc;
   KW_LAUNCH(launch_adj_scatter(g, P->nb, P->max_sens, af, P->d_inst, t, P->st));
+  if (af.absorb == 2) {  // power law: tau w and eta w through the fractional column passes
+    KW_LAUNCH(launch_adj_abs1(g, P->nb, P->fl, af, P->d_inst, twz, P->st));
+    std::vector<XJob> fj = {{0, 0, XK_FRAC1, 0}};
+    std::vector<int> fb = {0};
+    if (af.disp) { fj.push_back({1, 1, XK_FRAC2, 0}); fb.push_back(1); }
+    if ((rc = ypass(P, 0, fb))) return rc;
+    if ((rc = xpass(P, fj, KK_XFRAC))) return rc;
+    if ((rc = ypass(P, 1, fb))) return rc;
+  }
   KW_LAUNCH(launch_adj_eos(g, P->nb, P->fl, af, P->d_inst, twz, t, P->st));
   bool src_on = false, filt = false;
   for (int b = 0; b < P->nb; ++b) {
@@ -1042,7 +1051,7 @@ int adjoint_step(kw_plan* P, const AdjFields& af, float* Zbuf, int t) {
 
 int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   if (!P || !adj || !grad_c || ckpt < 1) return fail(KW_EINVAL, "NULL argument or checkpoint interval < 1");
-  if (P->flags) return fail(KW_EUNSUPPORTED, "adjoint: linear lossless plans only");
+  if (P->flags & KW_NONLINEAR) return fail(KW_EUNSUPPORTED, "adjoint: linear plans only (no B/A)");
   if (generic_z(P->g) || P->fl.gstash)
     return fail(KW_EUNSUPPORTED, "adjoint: grids whose axis lengths (or odd row count) need the generic transforms are not differentiated");
   const Geom& g = P->g;
@@ -1081,6 +1090,16 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
     return rc;
   for (int i = 0; i < D; ++i)
     if ((rc = tmp.alloc(&af.ub[i], (size_t)nf)) || (rc = tmp.alloc(&af.rb[i], (size_t)nf))) return rc;
+  // absorbing media: the replay keeps each step's absorption / dispersion operands
+  af.absorb = (P->flags & KW_STOKES) ? 1 : (P->flags & KW_POWERLAW_ABS) ? 2 : 0;
+  if ((P->flags & KW_POWERLAW_DISP) && !(P->flags & KW_POWERLAW_ABS))
+    return fail(KW_EUNSUPPORTED, "adjoint: dispersion without absorption is not differentiated");
+  af.disp = (P->flags & KW_POWERLAW_DISP) ? 1 : 0;
+  af.ym1 = (float)(P->alpha_power - 1.0);
+  af.y = (float)P->alpha_power;
+  float *abuf = nullptr, *bbuf = nullptr;
+  if (af.absorb && (rc = tmp.alloc(&abuf, (size_t)std::min(ckpt, nt) * nf))) return rc;
+  if (af.disp && (rc = tmp.alloc(&bbuf, (size_t)std::min(ckpt, nt) * nf))) return rc;
   af.adj = dadj;
   af.nrec = nrec;
   af.nsens = I0.nsens;
@@ -1122,13 +1141,18 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
     }
     if (launch_setint(tdev, nb, t0, P->st) != hipSuccess) { rc = fail(KW_EHIP, "adjoint: step counter reset failed"); break; }
     for (int t = t0; t < t1 && rc == KW_OK; ++t) {
+      P->fl.adjA = abuf ? abuf + (int64_t)(t - t0) * nf : nullptr;
+      P->fl.adjB = bbuf ? bbuf + (int64_t)(t - t0) * nf : nullptr;
       if ((rc = enqueue_step(P))) break;
       if (hipMemcpyAsync(pbuf + (int64_t)(t - t0) * nf, P->fl.p, sizeof(float) * nf, hipMemcpyDeviceToDevice, P->st) != hipSuccess)
         rc = fail(KW_EHIP, "adjoint: copy of p failed");
     }
+    P->fl.adjA = P->fl.adjB = nullptr;
     P->fl.tdev = nullptr;  // column passes of the reverse step must not skip instances at t = Nt
     for (int t = t1 - 1; t >= t0 && rc == KW_OK; --t) {
       af.pt = pbuf + (int64_t)(t - t0) * nf;
+      af.At = abuf ? abuf + (int64_t)(t - t0) * nf : nullptr;
+      af.Bt = bbuf ? bbuf + (int64_t)(t - t0) * nf : nullptr;
       rc = adjoint_step(P, af, Zbuf, t);
     }
   }

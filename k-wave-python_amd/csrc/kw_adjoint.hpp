@@ -15,16 +15,36 @@
 //   (x/y passes)    Dop_i on v_i
 //   k_adj_sum       pbar = sum_i Dop_i v_i
 //   k_adj_srcgrad   cbar += -(scale / c) sig(t) [K R] at the source points
+//
+// Absorbing media (linear): the equation of state gains c^2 (tau A - eta B),
+// A = L1[rho0 d] (Stokes: rho0 d), d = sum_i Dop_i u_i, B = L2[sum rho]; the forward
+// replay keeps A and B of every step (Fields::adjA/adjB).  With w = c^2 pbar:
+//   cbar += w ((y-1) tau A - y eta B) / c,  rbar_i += w - L2[eta w],
+//   and the gradient jobs act on b dt rho0 rbar_i - rho0 L1[tau w] (Stokes: tau w).
+// For the power law k_adj_abs1 forms tau w and eta w (forward z-FFT into spectra 0
+// and 1) and the speed term; the fractional column passes apply L1, L2; k_adj_eos
+// then reads both back.
 #pragma once
 #include "kw_kernels.hpp"
 
 namespace kw {
 
-template <int N, int ND>
+// ABS: 0 lossless, 1 Stokes, 2 power law (spectra 0 and 1 hold L1[tau w], L2[eta w])
+template <int N, int ND, int ABS>
 __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_eos(Geom g, Fields fl, AdjFields af, const Inst* __restrict__ inst,
                                                          const float2* __restrict__ tw_g, int t) {
   KW_ZPASS_PROLOGUE
   const Inst& I = inst[b];
+  float2 l1[ABS == 2 ? C::E : 1], l2[ABS == 2 ? C::E : 1];
+  if constexpr (ABS == 2) {
+    inv_to_regs<N>(lds, fl.spec[0] + b * g.nspec, tw, row0, rows, g.nhp, tid, l1);
+    if (af.disp) {
+      inv_to_regs<N>(lds, fl.spec[1] + b * g.nspec, tw, row0, rows, g.nhp, tid, l2);
+    } else {
+#pragma unroll
+      for (int m = 0; m < C::E; ++m) l2[m] = make_float2(0.f, 0.f);
+    }
+  }
   __shared__ float pmlz[N];
   for (int i_ = tid; i_ < N; i_ += C::NT) pmlz[i_] = gp(I.pml[ND - 1])[i_];
   __syncthreads();
@@ -45,12 +65,18 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_eos(Geom g, Fields fl, AdjF
     pbv[m] = make_float2(pb[i0], pb[i0 + N]);
     c2v[m] = make_float2(ld(I.c0sq, i0, I.c0sq_s), ld(I.c0sq, i0 + N, I.c0sq_s));
     rs[m] = make_float2(0.f, 0.f);
-    // p = c^2 sum rho  ->  dJ/dc += 2 c sum(rho) pbar = 2 p pbar / c
-    const float ga = 2.f * pt[i0] * pbv[m].x / sqrtf(c2v[m].x);
-    const float gb = 2.f * pt[i0 + N] * pbv[m].y / sqrtf(c2v[m].y);
-    if (inside) {
-      cb[i0] = cb[i0] + ga;
-      cb[i0 + N] = cb[i0 + N] + gb;
+    if constexpr (ABS != 2) {  // the power law's speed terms are k_adj_abs1's
+      // p = c^2 sum rho  ->  dJ/dc += 2 c sum(rho) pbar = 2 p pbar / c
+      float ga = 2.f * pt[i0] * pbv[m].x / sqrtf(c2v[m].x);
+      float gb = 2.f * pt[i0 + N] * pbv[m].y / sqrtf(c2v[m].y);
+      if constexpr (ABS == 1) {  // Stokes: tau = -2 a c, d tau/dc = tau / c, operand A = rho0 d
+        ga += c2v[m].x * pbv[m].x * tau_at(I, i0, c2v[m].x) / sqrtf(c2v[m].x) * af.At[off + i0];
+        gb += c2v[m].y * pbv[m].y * tau_at(I, i0 + N, c2v[m].y) / sqrtf(c2v[m].y) * af.At[off + i0 + N];
+      }
+      if (inside) {
+        cb[i0] = cb[i0] + ga;
+        cb[i0 + N] = cb[i0 + N] + gb;
+      }
     }
   }
 #pragma unroll
@@ -65,9 +91,17 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_eos(Geom g, Fields fl, AdjF
       for (int h = 0; h < 2; ++h) {
         const int64_t idx = iA + h * N + m * C::TPL;
         const float s = sz * ph[i][h];
-        const float rbt = rb[idx] + (h ? c2v[m].y : c2v[m].x) * (h ? pbv[m].y : pbv[m].x);
+        const float wc = (h ? c2v[m].y : c2v[m].x) * (h ? pbv[m].y : pbv[m].x);  // c^2 pbar
+        float rbt = rb[idx] + wc;
+        float dbar = 0.f;  // adjoint of div u through the absorption term, over rho0
+        if constexpr (ABS == 1) dbar = tau_at(I, idx, h ? c2v[m].y : c2v[m].x) * wc;
+        if constexpr (ABS == 2) {
+          rbt -= h ? l2[m].y : l2[m].x;
+          dbar = h ? l1[m].y : l1[m].x;
+        }
         if (h) rs[m].y += src.fac[i] * rbt; else rs[m].x += src.fac[i] * rbt;
-        wv[h] = s * I.dt * ld(I.rho0, idx, I.rho0_s) * rbt;
+        const float r0 = ld(I.rho0, idx, I.rho0_s);
+        wv[h] = s * I.dt * r0 * rbt - r0 * dbar;
         rn[h] = s * s * rbt;
       }
       if (inside) {
@@ -86,6 +120,44 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_eos(Geom g, Fields fl, AdjF
       R[iA + N + m * C::TPL] = rs[m].y;
     }
   }
+}
+
+// Power-law pre-pass of a reverse step: w = c^2 pbar,
+//   cbar += 2 p pbar / c + w ((y-1) tau A - y eta B) / c,
+// and the forward z-FFTs of tau w (spectrum 0) and eta w (spectrum 1, with dispersion).
+template <int N, int ND>
+__global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_abs1(Geom g, Fields fl, AdjFields af, const Inst* __restrict__ inst,
+                                                          const float2* __restrict__ tw_g) {
+  KW_ZPASS_PROLOGUE
+  const Inst& I = inst[b];
+  KW_ROWS_CLAMPED
+  const int64_t off = b * g.ntot;
+  const auto pb = gp(af.pb + off);
+  const auto pt = gp(af.pt + off);
+  const auto cb = gw(af.cb + off);
+  float2 tw_v[C::E], ew_v[C::E];
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    float tv[2], ev[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t idx = iA + h * N + m * C::TPL;
+      const float c2 = ld(I.c0sq, idx, I.c0sq_s), c = sqrtf(c2);
+      const float pbar = pb[idx];
+      const float w = c2 * pbar;
+      const float tau = tau_at(I, idx, c2);
+      const float eta = af.disp ? eta_at(I, idx, c2) : 0.f;
+      float gc = 2.f * pt[idx] * pbar / c + w * af.ym1 * tau / c * af.At[off + idx];
+      if (af.disp) gc -= w * af.y * eta / c * af.Bt[off + idx];
+      if (inside) cb[idx] = cb[idx] + gc;
+      tv[h] = tau * w;
+      ev[h] = eta * w;
+    }
+    tw_v[m] = make_float2(tv[0], tv[1]);
+    ew_v[m] = make_float2(ev[0], ev[1]);
+  }
+  fwd_from_regs<N>(lds, tw_v, fl.spec[0] + b * g.nspec, tw, row0, rows, g.nhp, tid);
+  if (af.disp) fwd_from_regs<N>(lds, ew_v, fl.spec[1] + b * g.nspec, tw, row0, rows, g.nhp, tid);
 }
 
 template <int N>

@@ -29,6 +29,7 @@ KW_DECL4(hipError_t, launch_eos, const Geom&, int, const Fields&, const Inst*, c
 KW_DECL4(hipError_t, launch_col, const Geom&, int, const Fields&, const Ops&, const Inst*, const ColArgs&, int, hipStream_t)
 KW_DECL4(hipError_t, launch_adj_eos, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, int, hipStream_t)
 KW_DECL4(hipError_t, launch_adj_mom, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, hipStream_t)
+KW_DECL4(hipError_t, launch_adj_abs1, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, hipStream_t)
 KW_DECL4(hipError_t, launch_adj_sum, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, hipStream_t)
 
 hipError_t launch_zfwd(const Geom& g, int nb, const float* f, int64_t fs, float2* spec, int64_t ss, const float2* tw, hipStream_t st) {
@@ -69,6 +70,9 @@ hipError_t launch_adj_eos(const Geom& g, int nb, const Fields& fl, const AdjFiel
 }
 hipError_t launch_adj_mom(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st) {
   KW_TRY4(launch_adj_mom, g, nb, fl, af, inst, tw, st);
+}
+hipError_t launch_adj_abs1(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st) {
+  KW_TRY4(launch_adj_abs1, g, nb, fl, af, inst, tw, st);
 }
 hipError_t launch_adj_sum(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st) {
   KW_TRY4(launch_adj_sum, g, nb, fl, af, inst, tw, st);

@@ -528,7 +528,10 @@ __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MASS) void k_mass(Geo
       } else {
         float q = rtv;
         const float c2 = ld(I.c0sq, idx, I.c0sq_s);
-        if (flags & FLAG_STOKES) q += tau_at(I, idx, c2) * r0 * du;
+        if (flags & FLAG_STOKES) {
+          q += tau_at(I, idx, c2) * r0 * du;
+          if (fl.adjA && inside) gw(fl.adjA)[off + idx] = r0 * du;  // adjoint replay: the Stokes operand
+        }
         if (flags & FLAG_NONLINEAR) q += ld(I.bona, idx, I.bona_s) * rtv * rtv / (2.f * r0);
         a_out[h] = c2 * q;
       }
@@ -577,6 +580,20 @@ __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_EOS) void k_eos(Geom 
   float2 av[C::E], bv[C::E];
   if (flags & FLAG_POWERLAW_ABS) inv_to_regs<N>(lds, fl.spec[0] + b * g.nspec, tw, row0, rows, g.nhp, tid, av);
   if (flags & FLAG_POWERLAW_DISP) inv_to_regs<N>(lds, fl.spec[1] + b * g.nspec, tw, row0, rows, g.nhp, tid, bv);
+  if (inside && fl.adjA && (flags & FLAG_POWERLAW_ABS)) {  // adjoint replay: keep the operands
+#pragma unroll
+    for (int m = 0; m < C::E; ++m) {
+      gw(fl.adjA)[off + iA + m * C::TPL] = av[m].x;
+      gw(fl.adjA)[off + iA + N + m * C::TPL] = av[m].y;
+    }
+  }
+  if (inside && fl.adjB && (flags & FLAG_POWERLAW_DISP)) {
+#pragma unroll
+    for (int m = 0; m < C::E; ++m) {
+      gw(fl.adjB)[off + iA + m * C::TPL] = bv[m].x;
+      gw(fl.adjB)[off + iA + N + m * C::TPL] = bv[m].y;
+    }
+  }
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
     float out[2];

@@ -120,6 +120,11 @@ struct Fields {
   // contiguous-axis updates, and the stash of a column pass's forward transform
   float* tmp[4];
   float2* gstash;
+  // adjoint replay only (null otherwise): where the equation of state of the step
+  // writes its absorption operand (rho0 div u for Stokes, L1[rho0 div u] for the
+  // power law) and its dispersion operand L2[sum rho], [nb][ntot]
+  float* adjA;
+  float* adjB;
 };
 
 // State of the reverse-time sweep of kw_plan_adjoint_c (all [nb][ntot] unless noted).
@@ -132,6 +137,11 @@ struct AdjFields {
   const float* pt;   // forward p of the step being reversed
   const float* adj;  // dJ/d(recorded p), [nb][nrec][nsens]
   int nrec, nsens;
+  // absorbing media: 0 none, 1 Stokes, 2 power law; disp: the power law has dispersion
+  int absorb, disp;
+  float ym1, y;      // alpha_power - 1 and alpha_power
+  const float* At;   // forward absorption operand of the step being reversed (Fields::adjA)
+  const float* Bt;   // forward dispersion operand (Fields::adjB)
 };
 
 }  // namespace kw

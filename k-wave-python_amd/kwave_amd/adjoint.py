@@ -2,9 +2,10 @@
 
 The reference solver (kwave/solvers/kspace_solver.py) is forward-only.  This
 module differentiates its step (``Simulation.step``, :691-747) for linear
-lossless media with additive pressure sources and/or an initial pressure p0
-(whose t = 0 state, :729-733, depends on c0 through the initial densities
-p0 / (D c0^2)) and binary or Cartesian sensors: ``sound_speed_gradient`` returns dJ/dc0 for a given dJ/d(traces), and
+media - lossless, or with Stokes or power-law absorption and dispersion
+(:471-516; tau and eta depend on c0) - with additive pressure sources and/or an
+initial pressure p0 (whose t = 0 state, :729-733, depends on c0 through the
+initial densities p0 / (D c0^2)) and binary or Cartesian sensors: ``sound_speed_gradient`` returns dJ/dc0 for a given dJ/d(traces), and
 ``sensor_traces`` wraps forward + adjoint as a ``torch.autograd.Function`` so a
 misfit written in torch can be minimised over the sound-speed map.
 

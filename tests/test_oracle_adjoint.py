@@ -69,7 +69,11 @@ def _cells(case, c, n_rand=6, seed=3):
     return cells
 
 
-ABSORBING = {"powerlaw": (40.0, 1.5, None), "powerlaw-no-dispersion": (40.0, 1.1, "no_dispersion"), "stokes": (40.0, 2.0, None)}
+# (alpha_coeff [dB/(MHz^y cm)], y, alpha_mode).  The explicit step is stable for Stokes
+# absorption only while tau c^2 k_max^2 dt stays below about 2, which on these grids
+# (dx = 0.1 mm, CFL 0.3) means alpha_coeff below about 0.04 dB/(MHz^2 cm).
+ABSORBING = {"powerlaw": (40.0, 1.5, None), "powerlaw-no-dispersion": (40.0, 1.1, "no_dispersion"), "stokes": (0.02, 2.0, None),
+             "powerlaw-mild": (5.0, 1.5, None)}
 
 
 @pytest.mark.parametrize("mode,p0,alpha", [("additive", False, None), ("additive-no-correction", False, None), ("additive", True, None),
