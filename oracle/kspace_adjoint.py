@@ -44,6 +44,13 @@ w = c^2 pbar the reverse step gains
     ubar_i += Gop_i(b_i dt rho0 rbar_i - rho0 L1[tau w])   (Dop_i^T = -Gop_i)
 with A and B of each step kept from the forward sweep.
 
+Nonlinear media (B/A, kspace_solver.py:537-547): the equation of state gains
+c^2 B/A S^2 / (2 rho0), S = sum_i rho_i, and the mass update multiplies Dop_i u_i by
+(2 S^{t-1} + rho0) / rho0.  The reverse step then takes w (1 + B/A S^t / rho0) into
+rbar_i, multiplies the gradient-job input b_i dt rho0 rbar_i by that factor, and adds
+-2 dt sum_i b_i d_i^t rbar_i (d_i^t = Dop_i u_i^t kept from the forward sweep) to every
+rbar_j of step t-1.
+
 Pins: finite differences of OracleSimulation in FP64 (tests/test_oracle_adjoint.py).
 """
 from __future__ import annotations
@@ -80,8 +87,6 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
     """
     if sim.t != 0:
         raise ValueError("adjoint: pass a fresh simulation")
-    if sim.BonA is not None:
-        raise ValueError("adjoint: linear media only (no B/A)")
     if sim.cart is not None or sim.sensor_idx is None:
         raise ValueError("adjoint: binary sensor masks only")
     if any(_is_enabled(getattr(sim.source, n, 0)) for n in ("ux", "uy", "uz")):
@@ -89,13 +94,17 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
     nd, shape = sim.ndim, sim.shape
     adj_src = np.asarray(adj_src, dtype=np.float64)
     absorbing = sim.tau is not None or sim.eta is not None
-    ps, As, Bs = [], [], []
+    nonlinear = sim.BonA is not None
+    ps, As, Bs, Ss, Ds = [], [], [], [], []
     while sim.t < sim.Nt:
         sim.step()
         ps.append(sim.p.copy())
         if absorbing:  # the absorption and dispersion operands of this step's equation of state
             As.append(_absorption_operand(sim))
             Bs.append(sim._diff(sum(sim.rho), sim.nabla2) if sim.eta is not None else None)
+        if nonlinear:  # sum of the densities after the step, and each axis's Dop_i u_i
+            Ss.append(sum(sim.rho))
+            Ds.append([sim._diff(sim.u[i], sim.op_div[i]) for i in range(nd)])
     rec = sim.data["p"].copy()
     src = _source_points(sim)
     c0 = sim.c0 if np.ndim(sim.c0) else np.full(shape, float(sim.c0))
@@ -115,9 +124,12 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
         w = c0_sq * pbar
         cbar += 2 * ps[t] * pbar / c0
         dbar = 0.0
+        wc = w
         if absorbing:
             cbar += _absorption_speed_term(sim, c0, w, As[t], Bs[t])
             dbar, w = _absorption_adjoint(sim, rho0, w)
+        if nonlinear:
+            w = w + _nonlinear_eos_term(sim, rho0, wc, Ss[t])
         rbar = [rbar[i] + w for i in range(nd)]
         if src is not None:
             pts, sig, rows, base, c, mode = src
@@ -127,9 +139,15 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
                 flat = cbar.ravel()
                 flat[pts] += -(base / c) * sig[rows, t] * Z.ravel()[pts]
                 cbar = flat.reshape(shape)
+        nl = 1.0
+        sbar = 0.0
+        if nonlinear:  # the mass update of step t multiplies by (2 S^{t-1} + rho0) / rho0
+            s_prev = Ss[t - 1] if t > 0 else 0.0
+            nl = (2.0 * s_prev + rho0) / rho0
+            sbar = _nonlinear_mass_term(sim, rbar, Ds[t])
         for i in range(nd):
-            w = sim.pml[i] * sim.dt * rho0 * rbar[i] - dbar
-            rbar[i] = sim.pml[i] ** 2 * rbar[i]
+            w = sim.pml[i] * sim.dt * rho0 * nl * rbar[i] - dbar
+            rbar[i] = sim.pml[i] ** 2 * rbar[i] + sbar
             ubar[i] = ubar[i] + sim._diff(w, sim.op_grad[i])
         pnew = np.zeros(shape)
         for i in range(nd):
@@ -174,6 +192,19 @@ def _absorption_speed_term(sim, c0, w, A, B):
     if sim.eta is not None:
         out = out - w * y * sim.eta / c0 * B
     return out
+
+
+def _nonlinear_eos_term(sim, rho0, w, S):
+    """What each rbar_i gains from the B/A term c^2 B/A S^2 / (2 rho0) of the equation of
+    state (kspace_solver.py:546): w B/A S / rho0, w = c^2 pbar."""
+    return w * sim.BonA * S / rho0
+
+
+def _nonlinear_mass_term(sim, rbar, D):
+    """Adjoint of S^{t-1} through the factor (2 S^{t-1} + rho0) / rho0 of the mass update
+    rho_i^t = b_i (b_i rho_i^{t-1} - dt rho0 d_i nl) (kspace_solver.py:547, :713-719):
+    -2 dt sum_i b_i d_i rbar_i, added to every rbar_j of step t-1."""
+    return -2.0 * sim.dt * sum(sim.pml[i] * D[i] * rbar[i] for i in range(sim.ndim))
 
 
 def _initial_density_term(sim, c0, rbar):

@@ -12,7 +12,8 @@ import numpy as np
 import pytest
 
 
-def adjoint_case(shape=(32, 40), nt=90, seed=7, mode="additive", p0=False, line_source=True, alpha=None):
+def adjoint_case(shape=(32, 40), nt=90, seed=7, mode="additive", p0=False, line_source=True, alpha=None, bona=None,
+                 amplitude=1e4):
     from kwave_amd.data import Vector
     from kwave_amd.kgrid import kWaveGrid
     from kwave_amd.kmedium import kWaveMedium
@@ -28,6 +29,8 @@ def adjoint_case(shape=(32, 40), nt=90, seed=7, mode="additive", p0=False, line_
     c = 1500.0 + 150.0 * gaussian_filter(rng.random(shape), 2)
     rho = 1000.0 + 100.0 * gaussian_filter(rng.random(shape), 2)
     med = kWaveMedium(sound_speed=c, density=rho)
+    if bona is not None:
+        med.BonA = bona
     if alpha is not None:  # (alpha_coeff [dB/(MHz^y cm)], alpha_power, alpha_mode)
         med.alpha_coeff, med.alpha_power = alpha[0], alpha[1]
         if alpha[2]:
@@ -37,12 +40,12 @@ def adjoint_case(shape=(32, 40), nt=90, seed=7, mode="additive", p0=False, line_
         mask = np.zeros(shape, dtype=bool)
         mask[(8,) + tuple(slice(s // 2 - 3, s // 2 + 3) for s in shape[1:])] = True
         src.p_mask = mask
-        src.p = 1e4 * tone_burst(1.0 / kg.dt, 2.5e6, 2)
+        src.p = amplitude * tone_burst(1.0 / kg.dt, 2.5e6, 2)
         src.p_mode = mode
     if p0:  # an initial pressure blob beside the line source
         grid = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
         r2 = sum((g - s // 3) ** 2 for g, s in zip(grid, shape))
-        src.p0 = 2e3 * np.exp(-r2 / 8.0)
+        src.p0 = 0.2 * amplitude * np.exp(-r2 / 8.0)
     smask = np.zeros(shape, dtype=bool)
     smask[(shape[0] - 9,) + tuple(slice(s // 4, 3 * s // 4, 2) for s in shape[1:])] = True
     sen = kSensor(smask)
@@ -76,13 +79,19 @@ ABSORBING = {"powerlaw": (40.0, 1.5, None), "powerlaw-no-dispersion": (40.0, 1.1
              "powerlaw-mild": (5.0, 1.5, None)}
 
 
-@pytest.mark.parametrize("mode,p0,alpha", [("additive", False, None), ("additive-no-correction", False, None), ("additive", True, None),
-                                           ("additive", False, "powerlaw"), ("additive", False, "powerlaw-no-dispersion"),
-                                           ("additive", True, "stokes")])
-def test_oracle_adjoint_matches_finite_differences(mode, p0, alpha):
+# B/A 6 at a source amplitude that makes the nonlinear terms matter: 3e7 Pa puts
+# B/A S / rho0 at a few per cent near the source
+NONLINEAR = dict(bona=6.0, amplitude=3e7)
+
+
+@pytest.mark.parametrize("mode,p0,alpha,nonlin", [("additive", False, None, False), ("additive-no-correction", False, None, False),
+                                                  ("additive", True, None, False), ("additive", False, "powerlaw", False),
+                                                  ("additive", False, "powerlaw-no-dispersion", False), ("additive", True, "stokes", False),
+                                                  ("additive", False, None, True), ("additive", True, "powerlaw-mild", True)])
+def test_oracle_adjoint_matches_finite_differences(mode, p0, alpha, nonlin):
     from oracle.kspace_adjoint import sound_speed_gradient
 
-    case = adjoint_case(mode=mode, p0=p0, alpha=ABSORBING.get(alpha))
+    case = adjoint_case(mode=mode, p0=p0, alpha=ABSORBING.get(alpha), **(NONLINEAR if nonlin else {}))
     c = np.asarray(case[1].sound_speed, dtype=np.float64)
     rec0 = _oracle(case, c).run()["p"]
     d = rec0 * 0.7 + 0.05 * np.abs(rec0).max() * np.random.default_rng(5).standard_normal(rec0.shape)
@@ -182,6 +191,38 @@ def test_oracle_adjoint_planted_absorption_slips_are_caught(alpha):
             bad, _ = kspace_adjoint.sound_speed_gradient(_oracle(case, c), rec0 - d)
         finally:
             kspace_adjoint._absorption_speed_term, kspace_adjoint._absorption_adjoint = orig_term, orig_adj
+        diff = np.abs(bad - good).ravel() * (flat < flat.max() - 1.0)
+        cell = int(np.argmax(diff))
+        cp, cm = c.copy(), c.copy()
+        cp.ravel()[cell] += h
+        cm.ravel()[cell] -= h
+        fd = (J(cp) - J(cm)) / (2 * h)
+        assert abs(good.ravel()[cell] - fd) <= 1e-6 * abs(fd), (name, good.ravel()[cell], fd)
+        assert abs(bad.ravel()[cell] - fd) > 1e-3 * abs(fd), (name, bad.ravel()[cell], fd)
+
+
+def test_oracle_adjoint_planted_nonlinear_slips_are_caught():
+    """Dropping the B/A term of the equation of state or the adjoint of S^{t-1} through the
+    mass update's nonlinear factor fails the finite-difference check."""
+    from oracle import kspace_adjoint
+
+    case = adjoint_case(**NONLINEAR)
+    c = np.asarray(case[1].sound_speed, dtype=np.float64)
+    rec0 = _oracle(case, c).run()["p"]
+    d = 0.5 * rec0
+    J = lambda cc: 0.5 * float(np.sum((_oracle(case, cc).run()["p"] - d) ** 2))  # noqa: E731
+    good, _ = kspace_adjoint.sound_speed_gradient(_oracle(case, c), rec0 - d)
+    orig_eos, orig_mass = kspace_adjoint._nonlinear_eos_term, kspace_adjoint._nonlinear_mass_term
+    plants = {"eos term dropped": ("_nonlinear_eos_term", lambda sim, rho0, w, S: 0.0 * w),
+              "mass term dropped": ("_nonlinear_mass_term", lambda sim, rbar, D: 0.0)}
+    h = 1e-2
+    flat = c.ravel()
+    for name, (attr, fn) in plants.items():
+        setattr(kspace_adjoint, attr, fn)
+        try:
+            bad, _ = kspace_adjoint.sound_speed_gradient(_oracle(case, c), rec0 - d)
+        finally:
+            kspace_adjoint._nonlinear_eos_term, kspace_adjoint._nonlinear_mass_term = orig_eos, orig_mass
         diff = np.abs(bad - good).ravel() * (flat < flat.max() - 1.0)
         cell = int(np.argmax(diff))
         cp, cm = c.copy(), c.copy()
