@@ -43,7 +43,7 @@ STEP_MODEL_BYTES = 412.0 + 60.0 * 440.0 / 1500.0
 
 PROFILE = ROOT / "profiles" / "rocprof_3d_nonlinear_512.md"
 # kernel kind (per-kernel HIP-event timing) -> compiled kernel name in the rocprof summary
-ROCPROF_NAME = {"col_y": "kw::k_col<{n}, 16, false, 1, 0>", "z_mass": "kw::k_mass<{n}, 3, false>",
+ROCPROF_NAME = {"col_y": "kw::k_col<{n}, 16, false, 1, 0>", "z_mass": "kw::k_mass<{n}, 3, false, false>",
                 "z_momentum": "kw::k_momentum<{n}>", "z_eos": "kw::k_eos<{n}>"}
 
 

@@ -1051,7 +1051,6 @@ int adjoint_step(kw_plan* P, const AdjFields& af, float* Zbuf, int t) {
 
 int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   if (!P || !adj || !grad_c || ckpt < 1) return fail(KW_EINVAL, "NULL argument or checkpoint interval < 1");
-  if (P->flags & KW_NONLINEAR) return fail(KW_EUNSUPPORTED, "adjoint: linear plans only (no B/A)");
   if (generic_z(P->g) || P->fl.gstash)
     return fail(KW_EUNSUPPORTED, "adjoint: grids whose axis lengths (or odd row count) need the generic transforms are not differentiated");
   const Geom& g = P->g;
@@ -1080,7 +1079,9 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   const int tb = with_p0 ? 1 : 0;
   const int nck = (nt - tb + ckpt - 1) / ckpt;
   const int64_t nf = (int64_t)nb * g.ntot;
-  const int nstate = 2 * D + 1;  // p, u_i, rho_i
+  // B/A: the density sum of the step before enters the mass update, so it is state too
+  const bool nonlin = (P->flags & KW_NONLINEAR) != 0;
+  const int nstate = 2 * D + 1 + (nonlin ? 1 : 0);  // p, u_i, rho_i (, sum of rho_i)
   DevBuf tmp;                    // freed when the call returns
   float *ck, *pbuf, *Zbuf, *dadj;
   AdjFields af{};
@@ -1100,12 +1101,20 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   float *abuf = nullptr, *bbuf = nullptr;
   if (af.absorb && (rc = tmp.alloc(&abuf, (size_t)std::min(ckpt, nt) * nf))) return rc;
   if (af.disp && (rc = tmp.alloc(&bbuf, (size_t)std::min(ckpt, nt) * nf))) return rc;
+  // B/A: the replay keeps the density sum before and after every step (ckpt + 1 slots)
+  // and each step's per-axis Dop_i u_i
+  float *sbuf = nullptr, *dbuf = nullptr;
+  af.nonlin = nonlin ? 1 : 0;
+  if (nonlin && ((rc = tmp.alloc(&sbuf, (size_t)(std::min(ckpt, nt) + 1) * nf)) ||
+                 (rc = tmp.alloc(&dbuf, (size_t)std::min(ckpt, nt) * D * nf))))
+    return rc;
   af.adj = dadj;
   af.nrec = nrec;
   af.nsens = I0.nsens;
   std::vector<float*> state = {P->fl.p};
   for (int i = 0; i < D; ++i) state.push_back(P->fl.u[i]);
   for (int i = 0; i < D; ++i) state.push_back(P->fl.rho[i]);
+  if (nonlin) state.push_back(P->fl.rhot);
   auto copy_state = [&](int k, bool save) -> int {
     for (int s = 0; s < nstate; ++s) {
       float* slot = ck + ((int64_t)k * nstate + s) * nf;
@@ -1140,19 +1149,30 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
       break;
     }
     if (launch_setint(tdev, nb, t0, P->st) != hipSuccess) { rc = fail(KW_EHIP, "adjoint: step counter reset failed"); break; }
+    if (nonlin && hipMemcpyAsync(sbuf, P->fl.rhot, sizeof(float) * nf, hipMemcpyDeviceToDevice, P->st) != hipSuccess) {
+      rc = fail(KW_EHIP, "adjoint: copy of the density sum failed");
+      break;
+    }
     for (int t = t0; t < t1 && rc == KW_OK; ++t) {
       P->fl.adjA = abuf ? abuf + (int64_t)(t - t0) * nf : nullptr;
       P->fl.adjB = bbuf ? bbuf + (int64_t)(t - t0) * nf : nullptr;
+      P->fl.adjD = dbuf ? dbuf + (int64_t)(t - t0) * D * nf : nullptr;
       if ((rc = enqueue_step(P))) break;
       if (hipMemcpyAsync(pbuf + (int64_t)(t - t0) * nf, P->fl.p, sizeof(float) * nf, hipMemcpyDeviceToDevice, P->st) != hipSuccess)
         rc = fail(KW_EHIP, "adjoint: copy of p failed");
+      else if (nonlin && hipMemcpyAsync(sbuf + (int64_t)(t - t0 + 1) * nf, P->fl.rhot, sizeof(float) * nf, hipMemcpyDeviceToDevice,
+                                        P->st) != hipSuccess)
+        rc = fail(KW_EHIP, "adjoint: copy of the density sum failed");
     }
-    P->fl.adjA = P->fl.adjB = nullptr;
+    P->fl.adjA = P->fl.adjB = P->fl.adjD = nullptr;
     P->fl.tdev = nullptr;  // column passes of the reverse step must not skip instances at t = Nt
     for (int t = t1 - 1; t >= t0 && rc == KW_OK; --t) {
       af.pt = pbuf + (int64_t)(t - t0) * nf;
       af.At = abuf ? abuf + (int64_t)(t - t0) * nf : nullptr;
       af.Bt = bbuf ? bbuf + (int64_t)(t - t0) * nf : nullptr;
+      af.St = sbuf ? sbuf + (int64_t)(t - t0 + 1) * nf : nullptr;
+      af.Sp = sbuf ? sbuf + (int64_t)(t - t0) * nf : nullptr;
+      af.Dt = dbuf ? dbuf + (int64_t)(t - t0) * D * nf : nullptr;
       rc = adjoint_step(P, af, Zbuf, t);
     }
   }

@@ -24,6 +24,10 @@
 // For the power law k_adj_abs1 forms tau w and eta w (forward z-FFT into spectra 0
 // and 1) and the speed term; the fractional column passes apply L1, L2; k_adj_eos
 // then reads both back.
+//
+// B/A (nonlinear): rbar_i also gains w B/A S^t / rho0, the gradient-job input is
+// scaled by (2 S^{t-1} + rho0) / rho0, and -2 dt sum_i b_i d_i^t rbar_i goes to every
+// rbar_j of step t-1 (S and d_i = Dop_i u_i kept by the replay).
 #pragma once
 #include "kw_kernels.hpp"
 
@@ -79,6 +83,12 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_eos(Geom g, Fields fl, AdjF
       }
     }
   }
+  // B/A: the adjoint of S^{t-1} through the mass update's factor, -2 dt sum_i b_i d_i rbar_i,
+  // goes to every rbar_j of the step before (added after the axis loop)
+  float2 sb[C::E];
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) sb[m] = make_float2(0.f, 0.f);
+  const int64_t nf = (int64_t)gridDim.y * g.ntot;
 #pragma unroll
   for (int i = 0; i < ND; ++i) {
     const auto rb = gw(af.rb[i] + off);
@@ -92,7 +102,14 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_eos(Geom g, Fields fl, AdjF
         const int64_t idx = iA + h * N + m * C::TPL;
         const float s = sz * ph[i][h];
         const float wc = (h ? c2v[m].y : c2v[m].x) * (h ? pbv[m].y : pbv[m].x);  // c^2 pbar
+        const float r0 = ld(I.rho0, idx, I.rho0_s);
         float rbt = rb[idx] + wc;
+        float nl = 1.f;
+        if (af.nonlin) {  // EOS: c^2 B/A S^2 / (2 rho0);  mass: d_i (2 S^{t-1} + rho0) / rho0
+          const float bona = ld(I.bona, idx, I.bona_s);
+          rbt += wc * bona * af.St[off + idx] / r0;
+          nl = 1.f + 2.f * af.Sp[off + idx] / r0;
+        }
         float dbar = 0.f;  // adjoint of div u through the absorption term, over rho0
         if constexpr (ABS == 1) dbar = tau_at(I, idx, h ? c2v[m].y : c2v[m].x) * wc;
         if constexpr (ABS == 2) {
@@ -100,8 +117,11 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_eos(Geom g, Fields fl, AdjF
           dbar = h ? l1[m].y : l1[m].x;
         }
         if (h) rs[m].y += src.fac[i] * rbt; else rs[m].x += src.fac[i] * rbt;
-        const float r0 = ld(I.rho0, idx, I.rho0_s);
-        wv[h] = s * I.dt * r0 * rbt - r0 * dbar;
+        if (af.nonlin) {
+          const float sv = -2.f * I.dt * s * af.Dt[(int64_t)i * nf + off + idx] * rbt;
+          if (h) sb[m].y += sv; else sb[m].x += sv;
+        }
+        wv[h] = s * I.dt * r0 * nl * rbt - r0 * dbar;
         rn[h] = s * s * rbt;
       }
       if (inside) {
@@ -111,6 +131,17 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_eos(Geom g, Fields fl, AdjF
       w[m] = make_float2(wv[0], wv[1]);
     }
     fwd_from_regs<N>(lds, w, fl.spec[i] + b * g.nspec, tw, row0, rows, g.nhp, tid);
+  }
+  if (af.nonlin && inside) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      const auto rb = gw(af.rb[i] + off);
+#pragma unroll
+      for (int m = 0; m < C::E; ++m) {
+        rb[iA + m * C::TPL] += sb[m].x;
+        rb[iA + N + m * C::TPL] += sb[m].y;
+      }
+    }
   }
   if (srcon && inside) {
     const auto R = gw(af.R + off);

@@ -399,8 +399,11 @@ __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MOM) void k_momentum(
 // the dense filtered pattern) and added to every split density from there.
 // The other instantiation serves plans without such a source and the steps
 // after the signal has ended.
+// CAP: the adjoint replay of a nonlinear plan also keeps each axis's Dop_i u_i
+// (Fields::adjD).  A template switch rather than a runtime test, so the
+// stepping kernels' code, and with it their FP32 contractions, stays unchanged.
 // ---------------------------------------------------------------------------
-template <int N, int ND, bool SREG>
+template <int N, int ND, bool SREG, bool CAP>
 __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MASS) void k_mass(Geom g, Fields fl, const Inst* __restrict__ inst, const float2* __restrict__ tw_g, int flags) {
   KW_ZPASS_PROLOGUE
   const Inst& I = inst[b];
@@ -459,6 +462,16 @@ __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MASS) void k_mass(Geo
     }
     float2 d[C::E];
     inv_to_regs<N>(lds, fl.spec[i] + b * g.nspec, tw, row0, rows, g.nhp, tid, d);
+    if constexpr (CAP) {
+      if (inside) {
+        const auto dd = gw(fl.adjD + (int64_t)i * gridDim.y * g.ntot + off);
+#pragma unroll
+        for (int m = 0; m < C::E; ++m) {
+          dd[iA + m * C::TPL] = d[m].x;
+          dd[iA + N + m * C::TPL] = d[m].y;
+        }
+      }
+    }
 #pragma unroll
     for (int m = 0; m < C::E; ++m) {
       const float sz = (i == ND - 1) ? pmlz[j0 + m * C::TPL] : 1.f;

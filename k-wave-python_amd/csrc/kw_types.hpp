@@ -125,6 +125,7 @@ struct Fields {
   // power law) and its dispersion operand L2[sum rho], [nb][ntot]
   float* adjA;
   float* adjB;
+  float* adjD;  // per-axis Dop_i u_i of the step, [nd][nb][ntot] (nonlinear adjoint replay)
 };
 
 // State of the reverse-time sweep of kw_plan_adjoint_c (all [nb][ntot] unless noted).
@@ -142,6 +143,12 @@ struct AdjFields {
   float ym1, y;      // alpha_power - 1 and alpha_power
   const float* At;   // forward absorption operand of the step being reversed (Fields::adjA)
   const float* Bt;   // forward dispersion operand (Fields::adjB)
+  // B/A: sum of the densities after the step (St) and after the step before (Sp), and
+  // the step's per-axis Dop_i u_i ([nd][nb][ntot], Fields::adjD)
+  int nonlin;
+  const float* St;
+  const float* Sp;
+  const float* Dt;
 };
 
 }  // namespace kw

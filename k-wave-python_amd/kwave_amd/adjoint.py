@@ -1,17 +1,19 @@
 """Gradient of recorded pressure traces with respect to sound speed (reverse-time adjoint on the GPU).
 
 The reference solver (kwave/solvers/kspace_solver.py) is forward-only.  This
-module differentiates its step (``Simulation.step``, :691-747) for linear
-media - lossless, or with Stokes or power-law absorption and dispersion
-(:471-516; tau and eta depend on c0) - with additive pressure sources and/or an
-initial pressure p0 (whose t = 0 state, :729-733, depends on c0 through the
-initial densities p0 / (D c0^2)) and binary or Cartesian sensors: ``sound_speed_gradient`` returns dJ/dc0 for a given dJ/d(traces), and
+module differentiates its step (``Simulation.step``, :691-747) for lossless
+media, media with Stokes or power-law absorption and dispersion (:471-516; tau
+and eta depend on c0) and B/A nonlinearity (:542-550), with or without absorption,
+with additive pressure sources and/or an initial pressure p0 (whose t = 0 state,
+:729-733, depends on c0 through the initial densities p0 / (D c0^2)) and binary or
+Cartesian sensors: ``sound_speed_gradient`` returns dJ/dc0 for a given dJ/d(traces), and
 ``sensor_traces`` wraps forward + adjoint as a ``torch.autograd.Function`` so a
 misfit written in torch can be minimised over the sound-speed map.
 
 The forward state is checkpointed every ``checkpoint`` steps and replayed one
 segment at a time (memory: (2D+1) grids per checkpoint plus ``checkpoint``
-pressure grids).  c_ref = max(c0) (kspace_solver.py:231), dt, the PML and kappa
+pressure grids; absorbing media keep one or two more grids per replayed step, B/A
+one more grid per checkpoint and D+1 more per replayed step).  c_ref = max(c0) (kspace_solver.py:231), dt, the PML and kappa
 are held fixed: the gradient is exact for perturbations that keep max(c0).
 """
 from __future__ import annotations

@@ -6,8 +6,9 @@ adjoint source is dJ/dy = y - d.  Checks:
 * the GPU gradient equals the FP64 CPU adjoint (oracle/kspace_adjoint.py, itself
   pinned by finite differences in tests/test_oracle_adjoint.py) to FP32 accuracy,
   in 2-D with both additive source modes and in 3-D, with an initial pressure p0
-  beside the line source and with p0 alone, and in power-law (with and without
-  dispersion) and Stokes media;
+  beside the line source and with p0 alone, in power-law (with and without
+  dispersion) and Stokes media, and with B/A nonlinearity (alone, and with p0 and
+  power-law absorption);
 * central finite differences of the GPU solver at 20 seeded cells (seed 7) of the
   128^2 surrogate problem, 600 steps, in the region the wave reaches and returns
   from.  h = 5 m/s: the FP32 solver's rounding noise in J (~1e4 of J ~ 1e13 over
@@ -22,7 +23,7 @@ import copy
 import numpy as np
 import pytest
 
-from test_oracle_adjoint import ABSORBING, adjoint_case
+from test_oracle_adjoint import ABSORBING, NONLINEAR, adjoint_case
 
 pytestmark = pytest.mark.gpu
 
@@ -79,6 +80,25 @@ def test_gpu_adjoint_matches_fp64_adjoint(shape, mode, p0, line, alpha):
     assert _rel(y, y_ref) < 1e-4
     err = _rel(g_gpu, g_ref)
     print(f"{shape} {mode} p0={p0} {alpha}: gradient rel-L2 vs FP64 adjoint {err:.2e}")
+    assert err < 1e-3, err
+
+
+@pytest.mark.parametrize("shape,p0,line,alpha", [
+    ((32, 40), False, True, None), ((32, 40), True, True, "powerlaw-mild"), ((16, 24, 32), True, False, None),
+    ((32, 40), False, True, "stokes")])
+def test_gpu_adjoint_nonlinear_matches_fp64_adjoint(shape, p0, line, alpha):
+    """B/A = 6 at a source amplitude of 30 MPa (B/A S / rho0 reaches a few per cent near the
+    source): the equation of state's B/A term and the mass update's (2 S + rho0) / rho0 factor."""
+    case = adjoint_case(shape=shape, nt=90 if len(shape) == 2 else 60, p0=p0, line_source=line, alpha=ABSORBING.get(alpha),
+                        **NONLINEAR)
+    y = _gpu_traces(case)
+    d = _misfit_data(y)
+    g_gpu, y2 = _gpu_grad(case, y - d, checkpoint=16)
+    assert np.array_equal(y, y2)
+    g_ref, y_ref = _oracle_grad(case, y - d)
+    assert _rel(y, y_ref) < 1e-4
+    err = _rel(g_gpu, g_ref)
+    print(f"{shape} B/A p0={p0} {alpha}: gradient rel-L2 vs FP64 adjoint {err:.2e}")
     assert err < 1e-3, err
 
 

@@ -92,17 +92,24 @@ def test_looping_column_passes_bit_identical(shape, grid, monkeypatch):
         assert np.array_equal(outs[0][k], outs[1][k]), (shape, grid, k)
 
 
-@pytest.mark.parametrize("shape", [(80, 112), (40, 56, 48), (320, 64), (64, 448)])
+@pytest.mark.parametrize("shape", [(80, 112), (40, 56, 48), (320, 64), (448, 64)])
 def test_radix_plan_variants_agree_with_the_oracle(shape, monkeypatch):
     """Lengths with a factor 5 or 7 have two radix plans for the column passes (radix 2 or
     radix 4 next to the odd factor); the plan picks one by timing, KW_COL_PLANS forces it.
     Both transform the same lines to FP32 rounding (the two runs differ by rounding alone,
-    about 1e-5 after 50 nonlinear power-law steps), and both match the FP64 oracle."""
+    about 1e-5 after 50 nonlinear power-law steps), and both match the FP64 oracle.  The
+    sensor line sits 6 cells from the point source along the last axis, inside the ~13
+    cells the pulse covers in 50 steps: a line beyond that reach records only the
+    spectral tails of the point source, ~1e-6 of its peak, whose relative error is
+    rounding noise rather than a property of the transforms."""
     from test_gpu_parity import _compare, _run_oracle
 
     from kwave_amd.solvers.kspace_solver import Simulation
 
     kg, med, src, sen = _case(shape, nt=50)
+    mask = np.zeros(shape, dtype=bool)
+    mask[(slice(None),) * (len(shape) - 1) + (shape[-1] // 2 - 6,)] = True
+    sen.mask = mask
     pml = tuple(4 for _ in shape)
     alpha = tuple(2.0 for _ in shape)
     outs = []

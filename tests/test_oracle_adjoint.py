@@ -87,7 +87,8 @@ NONLINEAR = dict(bona=6.0, amplitude=3e7)
 @pytest.mark.parametrize("mode,p0,alpha,nonlin", [("additive", False, None, False), ("additive-no-correction", False, None, False),
                                                   ("additive", True, None, False), ("additive", False, "powerlaw", False),
                                                   ("additive", False, "powerlaw-no-dispersion", False), ("additive", True, "stokes", False),
-                                                  ("additive", False, None, True), ("additive", True, "powerlaw-mild", True)])
+                                                  ("additive", False, None, True), ("additive", True, "powerlaw-mild", True),
+                                                  ("additive", False, "stokes", True)])
 def test_oracle_adjoint_matches_finite_differences(mode, p0, alpha, nonlin):
     from oracle.kspace_adjoint import sound_speed_gradient
 
