@@ -1062,7 +1062,6 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
     const kw::Inst& I = P->h_inst[b];
     for (int i = 0; i < D; ++i)
       if (I.srcu[i].mode != SRC_NONE) return fail(KW_EUNSUPPORTED, "adjoint: velocity sources are not differentiated");
-    if (I.srcp.mode == SRC_DIRICHLET) return fail(KW_EUNSUPPORTED, "adjoint: dirichlet pressure sources are not differentiated");
     if (!I.rec_p) return fail(KW_EINVAL, "adjoint: every instance must record p");
     if (I.nt != I0.nt || I.nsens != I0.nsens || I.rec_start != I0.rec_start)
       return fail(KW_EINVAL, "adjoint: instances must share Nt, sensor count and record start");

@@ -1,4 +1,4 @@
-// Reverse-time (discrete adjoint) kernels of the linear lossless step, for the
+// Reverse-time (discrete adjoint) kernels of the step, for the
 // gradient of a sensor-trace functional with respect to the sound speed.
 //
 // Forward step t (kw_kernels.hpp; reference kspace_solver.py:691-747):
@@ -24,6 +24,10 @@
 // For the power law k_adj_abs1 forms tau w and eta w (forward z-FFT into spectra 0
 // and 1) and the speed term; the fractional column passes apply L1, L2; k_adj_eos
 // then reads both back.
+//
+// Dirichlet pressure source: at its points the step overwrote every rho_i with
+// fac_i sig / (D c^2), so cbar gains -2 fac_i v rbar_i / c there and rbar_i is
+// cleared before it is carried back through the mass update.
 //
 // B/A (nonlinear): rbar_i also gains w B/A S^t / rho0, the gradient-job input is
 // scaled by (2 S^{t-1} + rho0) / rho0, and -2 dt sum_i b_i d_i^t rbar_i goes to every
@@ -56,6 +60,7 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_eos(Geom g, Fields fl, AdjF
   const int64_t off = b * g.ntot;
   const Src& src = I.srcp;
   const bool srcon = src.mode != SRC_NONE && t < src.len;
+  const bool dir = srcon && src.mode == SRC_DIRICHLET;
   float ph[ND][2];
 #pragma unroll
   for (int i = 0; i < ND; ++i) pml_rows(I.pml, g, i, rA, ph[i]);
@@ -115,6 +120,13 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_eos(Geom g, Fields fl, AdjF
         if constexpr (ABS == 2) {
           rbt -= h ? l2[m].y : l2[m].x;
           dbar = h ? l1[m].y : l1[m].x;
+        }
+        if (dir) {  // dirichlet point: rho_i = fac_i v after the mass update, v = sig / (D c^2)
+          float v;
+          if (csr_lookup(src, rA + h, j0 + m * C::TPL, t, v)) {
+            if (inside) cb[idx] += -2.f * src.fac[i] * v * rbt / sqrtf(h ? c2v[m].y : c2v[m].x);
+            rbt = 0.f;  // the overwritten value carries no adjoint further back
+          }
         }
         if (h) rs[m].y += src.fac[i] * rbt; else rs[m].x += src.fac[i] * rbt;
         if (af.nonlin) {

@@ -4,7 +4,7 @@ The reference solver (kwave/solvers/kspace_solver.py) is forward-only.  This
 module differentiates its step (``Simulation.step``, :691-747) for lossless
 media, media with Stokes or power-law absorption and dispersion (:471-516; tau
 and eta depend on c0) and B/A nonlinearity (:542-550), with or without absorption,
-with additive pressure sources and/or an initial pressure p0 (whose t = 0 state,
+with additive or dirichlet pressure sources and/or an initial pressure p0 (whose t = 0 state,
 :729-733, depends on c0 through the initial densities p0 / (D c0^2)) and binary or
 Cartesian sensors: ``sound_speed_gradient`` returns dJ/dc0 for a given dJ/d(traces), and
 ``sensor_traces`` wraps forward + adjoint as a ``torch.autograd.Function`` so a
@@ -28,6 +28,8 @@ from .solvers.setup import build_instance
 
 
 def _plan(kgrid, medium, source, sensor, c, pml_size, pml_alpha, **kw):
+    if kw.get("source_scaled"):  # pre-scaled signals carry no c0 dependence the adjoint could use
+        raise ValueError("adjoint: source_scaled signals are not differentiated")
     med = copy.copy(medium)
     if c is not None:
         med.sound_speed = np.asarray(c, dtype=np.float64)

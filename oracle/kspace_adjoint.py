@@ -66,8 +66,6 @@ def _source_points(sim):
     if not (_is_enabled(mask_raw) and _is_enabled(sig_raw)):
         return None
     mode = getattr(src, "p_mode", None) or "additive"
-    if mode == "dirichlet":
-        raise ValueError("adjoint: dirichlet pressure sources are not differentiated")
     mask = np.array(mask_raw, dtype=bool).ravel()
     if mask.size == 1:
         mask = np.full(sim.c0.size, bool(mask[0]))
@@ -76,7 +74,9 @@ def _source_points(sim):
     sig = sig.reshape(1, -1) if sig.ndim == 1 else sig.reshape(-1, sig.shape[-1])
     rows = np.zeros(pts.size, dtype=int) if sig.shape[0] == 1 else np.arange(pts.size)
     c = sim.c0.ravel()[pts]
-    base = 2 * sim.dt / (sim.ndim * c)  # scale_i = base / d_i
+    # additive: scale_i = base / d_i (kspace_solver.py:619-624); dirichlet: the value written
+    # into every rho_i is sig base (:613-617)
+    base = 1.0 / (sim.ndim * c ** 2) if mode == "dirichlet" else 2 * sim.dt / (sim.ndim * c)
     return pts, sig, rows, base, c, mode
 
 
@@ -133,7 +133,12 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
         rbar = [rbar[i] + w for i in range(nd)]
         if src is not None:
             pts, sig, rows, base, c, mode = src
-            if t < sig.shape[1]:
+            if t < sig.shape[1] and mode == "dirichlet":
+                rbar, dc = _dirichlet_source_term(rbar, pts, base * sig[rows, t], c)
+                flat = cbar.ravel()
+                flat[pts] += dc
+                cbar = flat.reshape(shape)
+            elif t < sig.shape[1]:
                 R = sum(rbar[i] / sim.spacing[i] for i in range(nd))
                 Z = sim._diff(R, sim.source_kappa) if mode == "additive" else R
                 flat = cbar.ravel()
@@ -205,6 +210,20 @@ def _nonlinear_mass_term(sim, rbar, D):
     rho_i^t = b_i (b_i rho_i^{t-1} - dt rho0 d_i nl) (kspace_solver.py:547, :713-719):
     -2 dt sum_i b_i d_i rbar_i, added to every rbar_j of step t-1."""
     return -2.0 * sim.dt * sum(sim.pml[i] * D[i] * rbar[i] for i in range(sim.ndim))
+
+
+def _dirichlet_source_term(rbar, pts, v, c):
+    """A dirichlet pressure source writes v = sig / (D c^2) into every rho_i at its points
+    after the mass update (kspace_solver.py:86-91, :613-617): dJ/dc gains -2 v / c sum_i
+    rbar_i there, and the overwritten values carry no adjoint further back.  Returns
+    (rbar with the points cleared, the dJ/dc increments at the points)."""
+    dc = -2.0 * v / c * sum(r.ravel()[pts] for r in rbar)
+    out = []
+    for r in rbar:
+        f = r.ravel().copy()
+        f[pts] = 0.0
+        out.append(f.reshape(r.shape))
+    return out, dc
 
 
 def _initial_density_term(sim, c0, rbar):

@@ -5,7 +5,7 @@ adjoint source is dJ/dy = y - d.  Checks:
 
 * the GPU gradient equals the FP64 CPU adjoint (oracle/kspace_adjoint.py, itself
   pinned by finite differences in tests/test_oracle_adjoint.py) to FP32 accuracy,
-  in 2-D with both additive source modes and in 3-D, with an initial pressure p0
+  in 2-D with all three pressure-source modes and in 3-D, with an initial pressure p0
   beside the line source and with p0 alone, in power-law (with and without
   dispersion) and Stokes media, and with B/A nonlinearity (alone, and with p0 and
   power-law absorption);
@@ -64,10 +64,12 @@ def _misfit_data(y, seed=5):
     ((16, 24, 32), "additive", False, True, None), ((32, 40), "additive", True, True, None),
     ((32, 40), None, True, False, None), ((16, 24, 32), None, True, False, None),
     ((32, 40), "additive", False, True, "powerlaw"), ((32, 40), "additive", False, True, "powerlaw-no-dispersion"),
-    ((32, 40), "additive", True, True, "stokes"), ((16, 24, 32), "additive", False, True, "powerlaw-mild")])
+    ((32, 40), "additive", True, True, "stokes"), ((16, 24, 32), "additive", False, True, "powerlaw-mild"),
+    ((32, 40), "dirichlet", False, True, None), ((16, 24, 32), "dirichlet", True, True, "powerlaw-mild")])
 def test_gpu_adjoint_matches_fp64_adjoint(shape, mode, p0, line, alpha):
-    """Line source, initial pressure p0 beside it, or p0 alone (the initial-density term of
-    dJ/dc0); lossless, power-law (with and without dispersion) and Stokes media.  The 3-D
+    """Line source (additive, uncorrected or dirichlet), initial pressure p0 beside it, or p0
+    alone (the initial-density term of dJ/dc0); lossless, power-law (with and without
+    dispersion) and Stokes media.  The 3-D
     power-law case absorbs less: at 40 dB/(MHz^1.5 cm) the 16-cell grid keeps 17 % of the
     lossless amplitude and the FP32 forward traces sit 8.5e-4 from FP64."""
     case = adjoint_case(shape=shape, nt=90 if len(shape) == 2 else 60, mode=mode or "additive", p0=p0, line_source=line,

@@ -88,7 +88,8 @@ NONLINEAR = dict(bona=6.0, amplitude=3e7)
                                                   ("additive", True, None, False), ("additive", False, "powerlaw", False),
                                                   ("additive", False, "powerlaw-no-dispersion", False), ("additive", True, "stokes", False),
                                                   ("additive", False, None, True), ("additive", True, "powerlaw-mild", True),
-                                                  ("additive", False, "stokes", True)])
+                                                  ("additive", False, "stokes", True), ("dirichlet", False, None, False),
+                                                  ("dirichlet", True, "powerlaw-mild", True)])
 def test_oracle_adjoint_matches_finite_differences(mode, p0, alpha, nonlin):
     from oracle.kspace_adjoint import sound_speed_gradient
 
@@ -134,6 +135,38 @@ def test_oracle_adjoint_planted_sign_error_is_caught():
     J = lambda cc: 0.5 * float(np.sum((_oracle(case, cc).run()["p"] - d) ** 2))  # noqa: E731
     fd = (J(cp) - J(cm)) / (2 * h)
     assert abs(grad.ravel()[cell] - fd) > 1e-3 * abs(fd)
+
+
+@pytest.mark.parametrize("slip", ["speed term dropped", "points not cleared"])
+def test_oracle_adjoint_planted_dirichlet_slips_are_caught(slip):
+    """A dirichlet pressure source overwrites every rho_i at its points with sig / (D c^2):
+    dropping that value's c-derivative, or letting the overwritten densities keep their
+    adjoint, fails the finite-difference check at a source point."""
+    from oracle import kspace_adjoint
+
+    case = adjoint_case(mode="dirichlet")
+    c = np.asarray(case[1].sound_speed, dtype=np.float64)
+    rec0 = _oracle(case, c).run()["p"]
+    d = 0.5 * rec0
+    orig = kspace_adjoint._dirichlet_source_term
+
+    def planted(rbar, pts, v, cc):
+        rb, dc = orig(rbar, pts, v, cc)
+        return (rb, 0.0 * dc) if slip == "speed term dropped" else (rbar, dc)
+
+    try:
+        kspace_adjoint._dirichlet_source_term = planted
+        grad, _ = kspace_adjoint.sound_speed_gradient(_oracle(case, c), rec0 - d)
+    finally:
+        kspace_adjoint._dirichlet_source_term = orig
+    cell = int(np.where(np.asarray(case[2].p_mask).ravel())[0][2])
+    h = 1e-2
+    cp, cm = c.copy(), c.copy()
+    cp.ravel()[cell] += h
+    cm.ravel()[cell] -= h
+    J = lambda cc: 0.5 * float(np.sum((_oracle(case, cc).run()["p"] - d) ** 2))  # noqa: E731
+    fd = (J(cp) - J(cm)) / (2 * h)
+    assert abs(grad.ravel()[cell] - fd) > 1e-3 * abs(fd), (slip, grad.ravel()[cell], fd)
 
 
 def test_oracle_adjoint_planted_initial_density_slip_is_caught():
