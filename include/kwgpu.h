@@ -194,11 +194,12 @@ int kw_plan_destroy(kw_plan* plan);
  * initial pressure p0, of the initial split densities p0 / (c0^2 p0div_i) (the
  * state after step 0 is set from p0, kspace_solver.py:729-733, and only those
  * densities depend on c0), and in absorbing media the dependence of tau and eta
- * on c0 (kspace_solver.py:471-516), and for a dirichlet pressure source the
- * written value sig / (D c0^2) at its points (:613-617).  Plans with additive,
- * uncorrected additive or dirichlet pressure sources and/or p0, lossless or with Stokes or power-law
- * absorption (with or without dispersion), with or without B/A nonlinearity
- * (kspace_solver.py:542-550), no velocity sources; every
+ * on c0 (kspace_solver.py:471-516), for a dirichlet pressure source the
+ * written value sig / (D c0^2) at its points (:613-617), and for an additive
+ * velocity source its scale 2 c0 dt / d_i (:636-642).  Plans with additive,
+ * uncorrected additive or dirichlet pressure and velocity sources and/or p0,
+ * lossless or with Stokes or power-law absorption (with or without dispersion),
+ * with or without B/A nonlinearity (kspace_solver.py:542-550); every
  * instance records p with a common Nt, sensor count and record start.  The plan's fields
  * and records afterwards are those of the forward run's first segment replay:
  * call kw_plan_init before stepping it again.  The reference has no adjoint:

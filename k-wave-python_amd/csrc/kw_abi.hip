@@ -1030,7 +1030,7 @@ int adjoint_step(kw_plan* P, const AdjFields& af, float* Zbuf, int t) {
       if ((rc = ypass(P, 1, {D}))) return rc;
       KW_LAUNCH(launch_zinv(g, P->nb, P->fl.spec[D], g.nspec, Zbuf, g.ntot, twz, 1.f, P->st));
     }
-    KW_LAUNCH(launch_adj_srcgrad(g, P->nb, af, Zbuf, P->d_inst, t, P->st));
+    KW_LAUNCH(launch_adj_srcgrad(g, P->nb, af, -1, af.R, Zbuf, P->d_inst, t, P->st));
   }
   std::vector<int> bufs;
   std::vector<XJob> jobs;
@@ -1038,12 +1038,34 @@ int adjoint_step(kw_plan* P, const AdjFields& af, float* Zbuf, int t) {
   if ((rc = ypass(P, 0, bufs))) return rc;
   if ((rc = xpass(P, jobs, KK_XGRAD))) return rc;
   if ((rc = ypass(P, 1, bufs))) return rc;
-  KW_LAUNCH(launch_adj_mom(g, P->nb, P->fl, af, P->d_inst, twz, P->st));
+  KW_LAUNCH(launch_adj_mom(g, P->nb, P->fl, af, P->d_inst, twz, t, P->st));
   for (int i = 0; i < D; ++i) jobs[i].kind = XK_DIV;
   if ((rc = ypass(P, 0, bufs))) return rc;
   if ((rc = xpass(P, jobs, KK_XDIV))) return rc;
   if ((rc = ypass(P, 1, bufs))) return rc;
   KW_LAUNCH(launch_adj_sum(g, P->nb, P->fl, af, P->d_inst, twz, P->st));
+  // velocity sources: the speed term of each axis's additive source, from dJ/du_i^t that
+  // k_adj_mom kept in U_i (spectrum buffer D is free again once the divergence passes ran)
+  for (int ax = 0; ax < D; ++ax) {
+    if (!af.U[ax]) continue;
+    bool on = false, ufilt = false;
+    for (int b = 0; b < P->nb; ++b) {
+      const kw::Src& s = P->h_inst[b].srcu[ax];
+      if (s.mode != SRC_NONE && s.mode != SRC_DIRICHLET && t < s.len) {
+        on = true;
+        ufilt = ufilt || s.mode == SRC_PATTERN || s.mode == SRC_CHAIN;
+      }
+    }
+    if (!on) continue;
+    if (ufilt) {  // Z = F^-1[kappa_s F[U_i]]
+      KW_LAUNCH(launch_zfwd(g, P->nb, af.U[ax], g.ntot, P->fl.spec[D], g.nspec, twz, P->st));
+      if ((rc = ypass(P, 0, {D}))) return rc;
+      if ((rc = xpass(P, {{D, D, XK_SRCKAPPA, 0}}, KK_XDIV))) return rc;
+      if ((rc = ypass(P, 1, {D}))) return rc;
+      KW_LAUNCH(launch_zinv(g, P->nb, P->fl.spec[D], g.nspec, Zbuf, g.ntot, twz, 1.f, P->st));
+    }
+    KW_LAUNCH(launch_adj_srcgrad(g, P->nb, af, ax, af.U[ax], Zbuf, P->d_inst, t, P->st));
+  }
   return KW_OK;
 }
 
@@ -1060,8 +1082,6 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   for (int b = 0; b < nb; ++b) {
     if (!P->inst_set[b]) return fail(KW_EINVAL, "instance " + std::to_string(b) + " not set");
     const kw::Inst& I = P->h_inst[b];
-    for (int i = 0; i < D; ++i)
-      if (I.srcu[i].mode != SRC_NONE) return fail(KW_EUNSUPPORTED, "adjoint: velocity sources are not differentiated");
     if (!I.rec_p) return fail(KW_EINVAL, "adjoint: every instance must record p");
     if (I.nt != I0.nt || I.nsens != I0.nsens || I.rec_start != I0.rec_start)
       return fail(KW_EINVAL, "adjoint: instances must share Nt, sensor count and record start");
@@ -1090,6 +1110,12 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
     return rc;
   for (int i = 0; i < D; ++i)
     if ((rc = tmp.alloc(&af.ub[i], (size_t)nf)) || (rc = tmp.alloc(&af.rb[i], (size_t)nf))) return rc;
+  // axes with an additive velocity source keep dJ/du_i of the step being reversed
+  for (int i = 0; i < D; ++i) {
+    bool add = false;
+    for (int b = 0; b < nb; ++b) add = add || (P->h_inst[b].srcu[i].mode != SRC_NONE && P->h_inst[b].srcu[i].mode != SRC_DIRICHLET);
+    if (add && (rc = tmp.alloc(&af.U[i], (size_t)nf))) return rc;
+  }
   // absorbing media: the replay keeps each step's absorption / dispersion operands
   af.absorb = (P->flags & KW_STOKES) ? 1 : (P->flags & KW_POWERLAW_ABS) ? 2 : 0;
   if ((P->flags & KW_POWERLAW_DISP) && !(P->flags & KW_POWERLAW_ABS))

@@ -39,8 +39,8 @@ hipError_t KW_GNAME(launch_adj_abs1)(const Geom& g, int nb, const Fields& fl, co
 }
 
 hipError_t KW_GNAME(launch_adj_mom)(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw,
-                                    hipStream_t st, bool* handled) {
-#define KW_C(S) case S: hipLaunchKernelGGL(k_adj_mom<S>, zgrid(g, ZCfg<S>::L, nb, g.nd), dim3(ZCfg<S>::NT), 0, st, g, fl, af, inst, tw); break;
+                                    int t, hipStream_t st, bool* handled) {
+#define KW_C(S) case S: hipLaunchKernelGGL(k_adj_mom<S>, zgrid(g, ZCfg<S>::L, nb, g.nd), dim3(ZCfg<S>::NT), 0, st, g, fl, af, inst, tw, t); break;
   KW_ZDISPATCH(KW_C)
 #undef KW_C
   return hipGetLastError();

@@ -205,7 +205,7 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_abs1(Geom g, Fields fl, Adj
 
 template <int N>
 __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_mom(Geom g, Fields fl, AdjFields af, const Inst* __restrict__ inst,
-                                                         const float2* __restrict__ tw_g) {
+                                                         const float2* __restrict__ tw_g, int t) {
   KW_ZPASS_PROLOGUE
   const int ax = blockIdx.z;
   const Inst& I = inst[b];
@@ -216,6 +216,12 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_mom(Geom g, Fields fl, AdjF
   KW_ROWS_CLAMPED
   float2* sp = fl.spec[ax] + b * g.nspec;
   const auto ub = gw(af.ub[ax] + b * g.ntot);
+  // velocity source of this axis at step t: dirichlet points drop their adjoint; for an
+  // additive one dJ/du_i^t goes to U (its speed term needs K applied, k_adj_srcgrad)
+  const Src& us = I.srcu[ax];
+  const bool uon = us.mode != SRC_NONE && t < us.len;
+  const bool udir = uon && us.mode == SRC_DIRICHLET;
+  float* const Ub = uon && !udir && af.U[ax] ? af.U[ax] + b * g.ntot : nullptr;
   float ph[2];
   pml_rows(I.pmlsg, g, ax, rA, ph);
   float2 uo[C::E], dr[C::E];
@@ -234,7 +240,12 @@ __global__ __launch_bounds__(ZCfg<N>::NT) void k_adj_mom(Geom g, Fields fl, AdjF
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const float s = sz * ph[h];
-      const float u = (h ? uo[m].y : uo[m].x) + (h ? gv[m].y : gv[m].x);
+      float u = (h ? uo[m].y : uo[m].x) + (h ? gv[m].y : gv[m].x);
+      if (udir) {
+        float v;
+        if (csr_lookup(us, rA + h, j0 + m * C::TPL, t, v)) u = 0.f;
+      }
+      if (Ub && inside) Ub[iA + h * N + m * C::TPL] = u;
       vv[h] = s * (h ? dr[m].y : dr[m].x) * u;
       un[h] = s * s * u;
     }

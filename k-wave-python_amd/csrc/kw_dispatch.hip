@@ -28,7 +28,7 @@ KW_DECL4(hipError_t, launch_mass, const Geom&, int, const Fields&, const Inst*, 
 KW_DECL4(hipError_t, launch_eos, const Geom&, int, const Fields&, const Inst*, const float2*, int, hipStream_t)
 KW_DECL4(hipError_t, launch_col, const Geom&, int, const Fields&, const Ops&, const Inst*, const ColArgs&, int, hipStream_t)
 KW_DECL4(hipError_t, launch_adj_eos, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, int, hipStream_t)
-KW_DECL4(hipError_t, launch_adj_mom, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, hipStream_t)
+KW_DECL4(hipError_t, launch_adj_mom, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, int, hipStream_t)
 KW_DECL4(hipError_t, launch_adj_abs1, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, hipStream_t)
 KW_DECL4(hipError_t, launch_adj_sum, const Geom&, int, const Fields&, const AdjFields&, const Inst*, const float2*, hipStream_t)
 
@@ -68,8 +68,8 @@ hipError_t launch_col(const Geom& g, int nb, const Fields& fl, const Ops& ops, c
 hipError_t launch_adj_eos(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, int t, hipStream_t st) {
   KW_TRY4(launch_adj_eos, g, nb, fl, af, inst, tw, t, st);
 }
-hipError_t launch_adj_mom(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st) {
-  KW_TRY4(launch_adj_mom, g, nb, fl, af, inst, tw, st);
+hipError_t launch_adj_mom(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, int t, hipStream_t st) {
+  KW_TRY4(launch_adj_mom, g, nb, fl, af, inst, tw, t, st);
 }
 hipError_t launch_adj_abs1(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st) {
   KW_TRY4(launch_adj_abs1, g, nb, fl, af, inst, tw, st);

@@ -48,11 +48,12 @@ hipError_t launch_mode_contract(int B, int K, int md, int NX, const float2* X, c
                                 const int64_t ys[4], hipStream_t st);
 // reverse sweep (kw_adjoint.hpp)
 hipError_t launch_adj_scatter(const Geom& g, int nb, int max_sens, const AdjFields& af, const Inst* inst, int t, hipStream_t st);
-hipError_t launch_adj_srcgrad(const Geom& g, int nb, const AdjFields& af, const float* Z, const Inst* inst, int t, hipStream_t st);
+hipError_t launch_adj_srcgrad(const Geom& g, int nb, const AdjFields& af, int which, const float* Zn, const float* Zk, const Inst* inst, int t,
+                              hipStream_t st);
 hipError_t launch_adj_p0(int64_t n, int D, const float* p0, const float* c0sq, int c0sq_s, const float* const rb[3], const float w[3],
                          float* cb, hipStream_t st);
 hipError_t launch_adj_eos(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, int t, hipStream_t st);
-hipError_t launch_adj_mom(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st);
+hipError_t launch_adj_mom(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, int t, hipStream_t st);
 hipError_t launch_adj_abs1(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st);
 hipError_t launch_adj_sum(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st);
 }  // namespace kw

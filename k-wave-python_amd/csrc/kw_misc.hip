@@ -79,23 +79,27 @@ __global__ void k_adj_scatter(Geom g, AdjFields af, const Inst* __restrict__ ins
   }
 }
 
-// Source-scale term: the injected value scale * sig(t) with scale = 2 dt / (D c d)
-// (kspace_solver.py:619-624) depends on c at the source point, so
-// cbar += -(scale / c) sig(t) Z with Z = K R (additive) or R (uncorrected).
-// One thread per grid row walks the row's source points (CSR).
-__global__ void k_adj_srcgrad(Geom g, AdjFields af, const float* __restrict__ Zk, const Inst* __restrict__ inst, int t) {
+// Source-scale term: the injected value scale * sig(t) depends on c at the source
+// point.  Pressure (which = -1): scale = 2 dt / (D c d) (kspace_solver.py:619-624), so
+// cbar += -(scale / c) sig(t) Z with Z = K R (additive) or R (uncorrected);
+// velocity along axis which: scale = 2 c dt / d_i (:636-642), so cbar += (scale / c)
+// sig(t) Z with Z = K U_i or U_i.  Zn is the unfiltered array (R or U_i), Zk the
+// filtered one.  One thread per grid row walks the row's source points (CSR).
+__global__ void k_adj_srcgrad(Geom g, AdjFields af, int which, const float* __restrict__ Zn, const float* __restrict__ Zk,
+                              const Inst* __restrict__ inst, int t) {
   const int b = blockIdx.y;
   const Inst& I = inst[b];
-  const Src& s = I.srcp;
+  const Src& s = which < 0 ? I.srcp : I.srcu[which];
   if (s.mode == SRC_NONE || s.mode == SRC_DIRICHLET || t >= s.len) return;
-  const float* Z = (s.mode == SRC_NOCORR ? af.R : Zk) + b * g.ntot;
+  const float* Z = (s.mode == SRC_NOCORR ? Zn : Zk) + b * g.ntot;
+  const float sgn = which < 0 ? -1.f : 1.f;
   float* cb = af.cb + b * g.ntot;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < g.rows; r += gridDim.x * blockDim.x) {
     for (int e = s.row_ptr[r]; e < s.row_ptr[r + 1]; ++e) {
       const int64_t idx = (int64_t)r * g.nz + s.col[e];
       const float c = sqrtf(I.c0sq[idx * I.c0sq_s]);
       const float sv = s.sig[(int64_t)s.sigrow[e] * s.len + t];
-      cb[idx] += -(s.scale[e] / c) * sv * Z[idx];
+      cb[idx] += sgn * (s.scale[e] / c) * sv * Z[idx];
     }
   }
 }
@@ -129,8 +133,9 @@ hipError_t launch_adj_scatter(const Geom& g, int nb, int max_sens, const AdjFiel
   return hipGetLastError();
 }
 
-hipError_t launch_adj_srcgrad(const Geom& g, int nb, const AdjFields& af, const float* Z, const Inst* inst, int t, hipStream_t st) {
-  hipLaunchKernelGGL(k_adj_srcgrad, dim3((g.rows + 255) / 256, nb), dim3(256), 0, st, g, af, Z, inst, t);
+hipError_t launch_adj_srcgrad(const Geom& g, int nb, const AdjFields& af, int which, const float* Zn, const float* Zk, const Inst* inst, int t,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_srcgrad, dim3((g.rows + 255) / 256, nb), dim3(256), 0, st, g, af, which, Zn, Zk, inst, t);
   return hipGetLastError();
 }
 

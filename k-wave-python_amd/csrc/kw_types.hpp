@@ -135,6 +135,7 @@ struct AdjFields {
   float* rb[3];      // adjoint of rho_i
   float* cb;         // accumulated dJ/dc0
   float* R;          // sum_i rb_i * fac_i on source-active steps
+  float* U[3];       // ub_i after the step's mass-update adjoint (dJ/du_i^t), axes with an additive velocity source
   const float* pt;   // forward p of the step being reversed
   const float* adj;  // dJ/d(recorded p), [nb][nrec][nsens]
   int nrec, nsens;

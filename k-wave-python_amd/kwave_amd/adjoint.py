@@ -4,7 +4,7 @@ The reference solver (kwave/solvers/kspace_solver.py) is forward-only.  This
 module differentiates its step (``Simulation.step``, :691-747) for lossless
 media, media with Stokes or power-law absorption and dispersion (:471-516; tau
 and eta depend on c0) and B/A nonlinearity (:542-550), with or without absorption,
-with additive or dirichlet pressure sources and/or an initial pressure p0 (whose t = 0 state,
+with additive or dirichlet pressure and velocity sources and/or an initial pressure p0 (whose t = 0 state,
 :729-733, depends on c0 through the initial densities p0 / (D c0^2)) and binary or
 Cartesian sensors: ``sound_speed_gradient`` returns dJ/dc0 for a given dJ/d(traces), and
 ``sensor_traces`` wraps forward + adjoint as a ``torch.autograd.Function`` so a

@@ -4,8 +4,8 @@ TEST INFRASTRUCTURE ONLY, like ``oracle/kspace_ref.py``: it is the checker of th
 HIP adjoint; only ``tests/`` import it.
 
 The reference has no adjoint; this differentiates its forward step
-(kwave/solvers/kspace_solver.py:691-747, restated in OracleSimulation.step) for
-the linear lossless case with an additive pressure source and a binary sensor:
+(kwave/solvers/kspace_solver.py:691-747, restated in OracleSimulation.step) with a
+binary sensor; first the linear lossless case with an additive pressure source:
 
     u_i^t = a_i^2 u_i^{t-1} - a_i beta_i Gop_i p^{t-1}          (:702-709)
     rho_i^t = b_i^2 rho_i^{t-1} - b_i dt rho0 Dop_i u_i^t + s_i(t) (:711-721)
@@ -51,6 +51,13 @@ rbar_i, multiplies the gradient-job input b_i dt rho0 rbar_i by that factor, and
 -2 dt sum_i b_i d_i^t rbar_i (d_i^t = Dop_i u_i^t kept from the forward sweep) to every
 rbar_j of step t-1.
 
+Dirichlet pressure sources write sig / (D c^2) into every rho_i at their points after
+the mass update: cbar gains -2 v / c sum_i rbar_i there and rbar_i is cleared.
+Velocity sources (kspace_solver.py:628-645, 707) act on u_i^t after the momentum
+update: additive ones add K[scale sig] with scale = 2 c dt / d_i, so cbar[pts] gains
+(scale / c) sig(t) [K ubar_i][pts] with ubar_i = dJ/du_i^t; dirichlet ones write sig
+(no c-dependence) and clear ubar_i at the points.
+
 Pins: finite differences of OracleSimulation in FP64 (tests/test_oracle_adjoint.py).
 """
 from __future__ import annotations
@@ -80,6 +87,46 @@ def _source_points(sim):
     return pts, sig, rows, base, c, mode
 
 
+def _velocity_source_points(sim):
+    """Per axis: (points, signal, signal rows, scale, c, mode) of the velocity source, or None.
+    additive: u_i += K[scale sig] with scale = 2 c dt / d_i; dirichlet: u_i = sig at the
+    points (kspace_solver.py:628-645)."""
+    src = sim.source
+    mask_raw = getattr(src, "u_mask", 0)
+    mode = getattr(src, "u_mode", None) or "additive"
+    out = []
+    for i, name in enumerate(["ux", "uy", "uz"][: sim.ndim]):
+        sig_raw = getattr(src, name, 0)
+        if not (_is_enabled(mask_raw) and _is_enabled(sig_raw)):
+            out.append(None)
+            continue
+        mask = np.array(mask_raw, dtype=bool).ravel()
+        if mask.size == 1:
+            mask = np.full(sim.c0.size, bool(mask[0]))
+        pts = np.where(mask)[0]
+        sig = np.array(sig_raw, dtype=np.float64)
+        sig = sig.reshape(1, -1) if sig.ndim == 1 else sig.reshape(-1, sig.shape[-1])
+        rows = np.zeros(pts.size, dtype=int) if sig.shape[0] == 1 else np.arange(pts.size)
+        c = sim.c0.ravel()[pts] if np.ndim(sim.c0) else np.full(pts.size, float(sim.c0))
+        scale = np.ones_like(c) if mode == "dirichlet" else 2 * c * sim.dt / sim.spacing[i]
+        out.append((pts, sig, rows, scale, c, mode))
+    return out if any(o is not None for o in out) else None
+
+
+def _velocity_source_term(sim, ubar_i, term, t):
+    """Adjoint of step t's velocity source on axis i, ubar_i = dJ/du_i^t: (ubar_i, the dJ/dc
+    increments at the points).  additive: d(scale)/dc = scale / c against K ubar_i
+    (uncorrected: ubar_i); dirichlet: the written value does not depend on c, and the
+    overwritten velocity carries no adjoint further back."""
+    pts, sig, rows, scale, c, mode = term
+    if mode == "dirichlet":
+        f = ubar_i.ravel().copy()
+        f[pts] = 0.0
+        return f.reshape(ubar_i.shape), 0.0
+    Z = sim._diff(ubar_i, sim.source_kappa) if mode == "additive" else ubar_i
+    return ubar_i, (scale / c) * sig[rows, t] * Z.ravel()[pts]
+
+
 def sound_speed_gradient(sim: OracleSimulation, adj_src):
     """Run ``sim`` forward from t = 0 and return (dJ/dc0 as a grid, recorded p).
 
@@ -89,8 +136,6 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
         raise ValueError("adjoint: pass a fresh simulation")
     if sim.cart is not None or sim.sensor_idx is None:
         raise ValueError("adjoint: binary sensor masks only")
-    if any(_is_enabled(getattr(sim.source, n, 0)) for n in ("ux", "uy", "uz")):
-        raise ValueError("adjoint: velocity sources are not differentiated")
     nd, shape = sim.ndim, sim.shape
     adj_src = np.asarray(adj_src, dtype=np.float64)
     absorbing = sim.tau is not None or sim.eta is not None
@@ -107,6 +152,7 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
             Ds.append([sim._diff(sim.u[i], sim.op_div[i]) for i in range(nd)])
     rec = sim.data["p"].copy()
     src = _source_points(sim)
+    usrc = _velocity_source_points(sim)
     c0 = sim.c0 if np.ndim(sim.c0) else np.full(shape, float(sim.c0))
     c0_sq = c0 ** 2
     rho0 = sim.rho0 if np.ndim(sim.rho0) else np.full(shape, float(sim.rho0))
@@ -154,6 +200,13 @@ def sound_speed_gradient(sim: OracleSimulation, adj_src):
             w = sim.pml[i] * sim.dt * rho0 * nl * rbar[i] - dbar
             rbar[i] = sim.pml[i] ** 2 * rbar[i] + sbar
             ubar[i] = ubar[i] + sim._diff(w, sim.op_grad[i])
+        if usrc is not None:  # ubar_i is now dJ/du_i^t, after step t's velocity source
+            for i, term in enumerate(usrc):
+                if term is not None and t < term[1].shape[1]:
+                    ubar[i], dc = _velocity_source_term(sim, ubar[i], term, t)
+                    flat = cbar.ravel()
+                    flat[term[0]] += dc
+                    cbar = flat.reshape(shape)
         pnew = np.zeros(shape)
         for i in range(nd):
             v = sim.pml_sg[i] * sim.dt_rho0[i] * ubar[i]

@@ -7,8 +7,8 @@ adjoint source is dJ/dy = y - d.  Checks:
   pinned by finite differences in tests/test_oracle_adjoint.py) to FP32 accuracy,
   in 2-D with all three pressure-source modes and in 3-D, with an initial pressure p0
   beside the line source and with p0 alone, in power-law (with and without
-  dispersion) and Stokes media, and with B/A nonlinearity (alone, and with p0 and
-  power-law absorption);
+  dispersion) and Stokes media, with B/A nonlinearity (alone, and with p0 and
+  power-law absorption), and with velocity sources in each mode;
 * central finite differences of the GPU solver at 20 seeded cells (seed 7) of the
   128^2 surrogate problem, 600 steps, in the region the wave reaches and returns
   from.  h = 5 m/s: the FP32 solver's rounding noise in J (~1e4 of J ~ 1e13 over
@@ -101,6 +101,26 @@ def test_gpu_adjoint_nonlinear_matches_fp64_adjoint(shape, p0, line, alpha):
     assert _rel(y, y_ref) < 1e-4
     err = _rel(g_gpu, g_ref)
     print(f"{shape} B/A p0={p0} {alpha}: gradient rel-L2 vs FP64 adjoint {err:.2e}")
+    assert err < 1e-3, err
+
+
+@pytest.mark.parametrize("shape,umode,p0,alpha,nonlin", [
+    ((32, 40), "additive", False, None, False), ((32, 40), "additive-no-correction", False, None, False),
+    ((32, 40), "dirichlet", False, None, False), ((16, 24, 32), "additive", True, "powerlaw-mild", True)])
+def test_gpu_adjoint_velocity_source_matches_fp64_adjoint(shape, umode, p0, alpha, nonlin):
+    """A velocity source on every axis beside the pressure line source: the speed term of the
+    additive source scale 2 c dt / d_i (K-filtered, or not), and a dirichlet source's
+    cleared points."""
+    case = adjoint_case(shape=shape, nt=90 if len(shape) == 2 else 60, p0=p0, alpha=ABSORBING.get(alpha), u_source=umode,
+                        **(NONLINEAR if nonlin else {}))
+    y = _gpu_traces(case)
+    d = _misfit_data(y)
+    g_gpu, y2 = _gpu_grad(case, y - d, checkpoint=16)
+    assert np.array_equal(y, y2)
+    g_ref, y_ref = _oracle_grad(case, y - d)
+    assert _rel(y, y_ref) < 1e-4
+    err = _rel(g_gpu, g_ref)
+    print(f"{shape} u:{umode} p0={p0} {alpha} B/A={nonlin}: gradient rel-L2 vs FP64 adjoint {err:.2e}")
     assert err < 1e-3, err
 
 

@@ -13,7 +13,7 @@ import pytest
 
 
 def adjoint_case(shape=(32, 40), nt=90, seed=7, mode="additive", p0=False, line_source=True, alpha=None, bona=None,
-                 amplitude=1e4):
+                 amplitude=1e4, u_source=None):
     from kwave_amd.data import Vector
     from kwave_amd.kgrid import kWaveGrid
     from kwave_amd.kmedium import kWaveMedium
@@ -46,6 +46,14 @@ def adjoint_case(shape=(32, 40), nt=90, seed=7, mode="additive", p0=False, line_
         grid = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
         r2 = sum((g - s // 3) ** 2 for g, s in zip(grid, shape))
         src.p0 = 0.2 * amplitude * np.exp(-r2 / 8.0)
+    if u_source:  # a velocity source (mode u_source) on a short line, every axis driven
+        umask = np.zeros(shape, dtype=bool)
+        umask[(12,) + tuple(slice(s // 2 - 2, s // 2 + 2) for s in shape[1:])] = True
+        src.u_mask = umask
+        usig = amplitude / (1000.0 * 1500.0) * tone_burst(1.0 / kg.dt, 2.5e6, 2)
+        for i, name in enumerate(["ux", "uy", "uz"][: len(shape)]):
+            setattr(src, name, (1.0 - 0.3 * i) * usig)
+        src.u_mode = u_source
     smask = np.zeros(shape, dtype=bool)
     smask[(shape[0] - 9,) + tuple(slice(s // 4, 3 * s // 4, 2) for s in shape[1:])] = True
     sen = kSensor(smask)
@@ -67,8 +75,10 @@ def _cells(case, c, n_rand=6, seed=3):
     flat = c.ravel()
     ok = np.where(flat < flat.max() - 1.0)[0]
     cells = list(rng.choice(ok, size=n_rand, replace=False))
-    src_pts = np.where(np.asarray(case[2].p_mask).ravel())[0]
-    cells += list(src_pts[[0, src_pts.size // 2]])
+    for name in ("p_mask", "u_mask"):
+        if getattr(case[2], name, None) is not None:
+            src_pts = np.where(np.asarray(getattr(case[2], name)).ravel())[0]
+            cells += list(src_pts[[0, src_pts.size // 2]])
     return cells
 
 
@@ -89,11 +99,16 @@ NONLINEAR = dict(bona=6.0, amplitude=3e7)
                                                   ("additive", False, "powerlaw-no-dispersion", False), ("additive", True, "stokes", False),
                                                   ("additive", False, None, True), ("additive", True, "powerlaw-mild", True),
                                                   ("additive", False, "stokes", True), ("dirichlet", False, None, False),
-                                                  ("dirichlet", True, "powerlaw-mild", True)])
+                                                  ("dirichlet", True, "powerlaw-mild", True), ("u:additive", False, None, False),
+                                                  ("u:additive-no-correction", False, None, False), ("u:dirichlet", False, None, False),
+                                                  ("u:additive", True, "powerlaw-mild", True)])
 def test_oracle_adjoint_matches_finite_differences(mode, p0, alpha, nonlin):
+    """mode "u:<m>": a velocity source in mode <m> beside the additive pressure line source."""
     from oracle.kspace_adjoint import sound_speed_gradient
 
-    case = adjoint_case(mode=mode, p0=p0, alpha=ABSORBING.get(alpha), **(NONLINEAR if nonlin else {}))
+    umode = mode[2:] if mode.startswith("u:") else None
+    case = adjoint_case(mode="additive" if umode else mode, p0=p0, alpha=ABSORBING.get(alpha), u_source=umode,
+                        **(NONLINEAR if nonlin else {}))
     c = np.asarray(case[1].sound_speed, dtype=np.float64)
     rec0 = _oracle(case, c).run()["p"]
     d = rec0 * 0.7 + 0.05 * np.abs(rec0).max() * np.random.default_rng(5).standard_normal(rec0.shape)
@@ -160,6 +175,35 @@ def test_oracle_adjoint_planted_dirichlet_slips_are_caught(slip):
     finally:
         kspace_adjoint._dirichlet_source_term = orig
     cell = int(np.where(np.asarray(case[2].p_mask).ravel())[0][2])
+    h = 1e-2
+    cp, cm = c.copy(), c.copy()
+    cp.ravel()[cell] += h
+    cm.ravel()[cell] -= h
+    J = lambda cc: 0.5 * float(np.sum((_oracle(case, cc).run()["p"] - d) ** 2))  # noqa: E731
+    fd = (J(cp) - J(cm)) / (2 * h)
+    assert abs(grad.ravel()[cell] - fd) > 1e-3 * abs(fd), (slip, grad.ravel()[cell], fd)
+
+
+@pytest.mark.parametrize("umode,slip", [("additive", "speed term dropped"), ("dirichlet", "points not cleared")])
+def test_oracle_adjoint_planted_velocity_source_slips_are_caught(umode, slip):
+    """Dropping the additive velocity source's scale derivative, or letting the velocity a
+    dirichlet source overwrote keep its adjoint, fails the finite-difference check at a
+    velocity-source point."""
+    from oracle import kspace_adjoint
+
+    case = adjoint_case(u_source=umode)
+    c = np.asarray(case[1].sound_speed, dtype=np.float64)
+    rec0 = _oracle(case, c).run()["p"]
+    d = 0.5 * rec0
+    orig = kspace_adjoint._velocity_source_term
+    planted = (lambda sim, ub, term, t: (ub, 0.0)) if slip == "speed term dropped" else \
+        (lambda sim, ub, term, t: (ub, orig(sim, ub, term, t)[1]))
+    try:
+        kspace_adjoint._velocity_source_term = planted
+        grad, _ = kspace_adjoint.sound_speed_gradient(_oracle(case, c), rec0 - d)
+    finally:
+        kspace_adjoint._velocity_source_term = orig
+    cell = int(np.where(np.asarray(case[2].u_mask).ravel())[0][1])
     h = 1e-2
     cp, cm = c.copy(), c.copy()
     cp.ravel()[cell] += h
