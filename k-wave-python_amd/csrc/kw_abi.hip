@@ -1073,8 +1073,6 @@ int adjoint_step(kw_plan* P, const AdjFields& af, float* Zbuf, int t) {
 
 int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   if (!P || !adj || !grad_c || ckpt < 1) return fail(KW_EINVAL, "NULL argument or checkpoint interval < 1");
-  if (generic_z(P->g) || P->fl.gstash)
-    return fail(KW_EUNSUPPORTED, "adjoint: grids whose axis lengths (or odd row count) need the generic transforms are not differentiated");
   const Geom& g = P->g;
   const int D = g.nd;
   const int nb = P->nb;
@@ -1126,6 +1124,8 @@ int kw_plan_adjoint_c(kw_plan* P, const float* adj, int ckpt, float* grad_c) {
   float *abuf = nullptr, *bbuf = nullptr;
   if (af.absorb && (rc = tmp.alloc(&abuf, (size_t)std::min(ckpt, nt) * nf))) return rc;
   if (af.disp && (rc = tmp.alloc(&bbuf, (size_t)std::min(ckpt, nt) * nf))) return rc;
+  // generic-length contiguous axis with the power law: L1[tau w] and L2[eta w] on the grid
+  if (generic_z(g) && af.absorb == 2 && ((rc = tmp.alloc(&af.gl[0], (size_t)nf)) || (rc = tmp.alloc(&af.gl[1], (size_t)nf)))) return rc;
   // B/A: the replay keeps the density sum before and after every step (ckpt + 1 slots)
   // and each step's per-axis Dop_i u_i
   float *sbuf = nullptr, *dbuf = nullptr;

@@ -66,15 +66,19 @@ hipError_t launch_col(const Geom& g, int nb, const Fields& fl, const Ops& ops, c
 }
 
 hipError_t launch_adj_eos(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, int t, hipStream_t st) {
+  if (generic_z(g)) return launch_adj_eos_gen(g, nb, fl, af, inst, tw, t, st);
   KW_TRY4(launch_adj_eos, g, nb, fl, af, inst, tw, t, st);
 }
 hipError_t launch_adj_mom(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, int t, hipStream_t st) {
+  if (generic_z(g)) return launch_adj_mom_gen(g, nb, fl, af, inst, tw, t, st);
   KW_TRY4(launch_adj_mom, g, nb, fl, af, inst, tw, t, st);
 }
 hipError_t launch_adj_abs1(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st) {
+  if (generic_z(g)) return launch_adj_abs1_gen(g, nb, fl, af, inst, tw, st);
   KW_TRY4(launch_adj_abs1, g, nb, fl, af, inst, tw, st);
 }
 hipError_t launch_adj_sum(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st) {
+  if (generic_z(g)) return launch_adj_sum_gen(g, nb, fl, af, inst, tw, st);
   KW_TRY4(launch_adj_sum, g, nb, fl, af, inst, tw, st);
 }
 

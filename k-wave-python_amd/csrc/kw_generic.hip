@@ -438,6 +438,7 @@ __global__ __launch_bounds__(kGenThreads) void k_gmass(Geom g, Fields fl, const 
     for (int i = 0; i < ND; ++i) {
       const auto rho = gw(fl.rho[i] + off);
       const float dv = gp(dd[i])[off + idx];
+      if (fl.adjD) gw(fl.adjD)[(int64_t)i * gridDim.y * g.ntot + off + idx] = dv;  // adjoint replay: Dop_i u_i
       const float s = gpml(I.pml, g, i, r, n);
       float rv = s * (s * rho[idx] - q * dv);
       if (smode == SRC_CHAIN || smode == SRC_PATTERN) rv = rv + src.fac[i] * ch;
@@ -465,7 +466,10 @@ __global__ __launch_bounds__(kGenThreads) void k_gmass(Geom g, Fields fl, const 
     } else {
       float qq = rt;
       const float c2 = ld(I.c0sq, idx, I.c0sq_s);
-      if (flags & FLAG_STOKES) qq += tau_at(I, idx, c2) * r0 * du;
+      if (flags & FLAG_STOKES) {
+        qq += tau_at(I, idx, c2) * r0 * du;
+        if (fl.adjA) gw(fl.adjA)[off + idx] = r0 * du;  // adjoint replay: the Stokes operand
+      }
       if (flags & FLAG_NONLINEAR) qq += ld(I.bona, idx, I.bona_s) * rt * rt / (2.f * r0);
       gw(fl.p)[off + idx] = c2 * qq;
     }
@@ -485,6 +489,9 @@ __global__ __launch_bounds__(kGenThreads) void k_geos(Geom g, Fields fl, const I
     float q = rt;
     if (flags & FLAG_POWERLAW_ABS) q += tau_at(I, idx, c2) * gp(av)[off + idx];
     if (flags & FLAG_POWERLAW_DISP) q -= eta_at(I, idx, c2) * gp(bv)[off + idx];
+    // adjoint replay: keep the absorption and dispersion operands
+    if (fl.adjA && (flags & FLAG_POWERLAW_ABS)) gw(fl.adjA)[off + idx] = gp(av)[off + idx];
+    if (fl.adjB && (flags & FLAG_POWERLAW_DISP)) gw(fl.adjB)[off + idx] = gp(bv)[off + idx];
     if (flags & FLAG_NONLINEAR) q += ld(I.bona, idx, I.bona_s) * rt * rt / (2.f * ld(I.rho0, idx, I.rho0_s));
     gw(fl.p)[off + idx] = c2 * q;
   }
@@ -528,6 +535,126 @@ __global__ __launch_bounds__(kGenThreads) void k_gsrc_scatter(Geom g, Fields fl,
       if (csr_lookup(s, r, n, t, x)) v = x;
     }
     gw(out)[off + idx] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reverse sweep: the arithmetic of kw_adjoint.hpp's kernels, one element per
+// thread; their transforms run as the launchers below (every instance, no step
+// filter: the reverse sweep runs all of them)
+// ---------------------------------------------------------------------------
+// power law: cbar += 2 p pbar / c + w ((y-1) tau A - y eta B) / c; tau w, eta w out
+__global__ __launch_bounds__(kGenThreads) void k_gadj_abs1(Geom g, AdjFields af, const Inst* __restrict__ inst,
+                                                           float* __restrict__ tout, float* __restrict__ eout) {
+  const int b = blockIdx.y;
+  const Inst& I = inst[b];
+  const int64_t off = (int64_t)b * g.ntot;
+  KW_GEN_ELEMENT_LOOP {
+    const float c2 = ld(I.c0sq, idx, I.c0sq_s), c = sqrtf(c2);
+    const float pbar = gp(af.pb)[off + idx];
+    const float w = c2 * pbar;
+    const float tau = tau_at(I, idx, c2);
+    const float eta = af.disp ? eta_at(I, idx, c2) : 0.f;
+    float gc = 2.f * gp(af.pt)[off + idx] * pbar / c + w * af.ym1 * tau / c * gp(af.At)[off + idx];
+    if (af.disp) gc -= w * af.y * eta / c * gp(af.Bt)[off + idx];
+    gw(af.cb)[off + idx] += gc;
+    gw(tout)[off + idx] = tau * w;
+    if (af.disp) gw(eout)[off + idx] = eta * w;
+  }
+}
+
+// equation of state, sources and mass update (k_adj_eos); l1, l2: L1[tau w], L2[eta w]
+// on the grid (power law), w_i out for the gradient jobs
+__global__ __launch_bounds__(kGenThreads) void k_gadj_eos(Geom g, AdjFields af, const Inst* __restrict__ inst, int t,
+                                                          const float* __restrict__ l1, const float* __restrict__ l2, float* w0,
+                                                          float* w1, float* w2) {
+  const int b = blockIdx.y;
+  const Inst& I = inst[b];
+  const int ND = g.nd;
+  const int64_t off = (int64_t)b * g.ntot;
+  const int64_t nf = (int64_t)gridDim.y * g.ntot;
+  const Src& src = I.srcp;
+  const bool srcon = src.mode != SRC_NONE && t < src.len;
+  const bool dir = srcon && src.mode == SRC_DIRICHLET;
+  float* const wo[3] = {w0, w1, w2};
+  KW_GEN_ELEMENT_LOOP {
+    const int r = (int)(idx / g.nz), n = (int)(idx % g.nz);
+    const float c2 = ld(I.c0sq, idx, I.c0sq_s);
+    const float pbar = gp(af.pb)[off + idx];
+    const float wc = c2 * pbar;
+    const float r0 = ld(I.rho0, idx, I.rho0_s);
+    float gc = 0.f;
+    if (af.absorb != 2) {  // the power law's speed terms are k_gadj_abs1's
+      gc = 2.f * gp(af.pt)[off + idx] * pbar / sqrtf(c2);
+      if (af.absorb == 1) gc += wc * tau_at(I, idx, c2) / sqrtf(c2) * gp(af.At)[off + idx];
+    }
+    float rs = 0.f, sb = 0.f, rn[3] = {0.f, 0.f, 0.f};
+    for (int i = 0; i < ND; ++i) {
+      const float s = gpml(I.pml, g, i, r, n);
+      float rbt = gp(af.rb[i])[off + idx] + wc;
+      float nl = 1.f;
+      if (af.nonlin) {
+        rbt += wc * ld(I.bona, idx, I.bona_s) * gp(af.St)[off + idx] / r0;
+        nl = 1.f + 2.f * gp(af.Sp)[off + idx] / r0;
+      }
+      float dbar = 0.f;
+      if (af.absorb == 1) dbar = tau_at(I, idx, c2) * wc;
+      if (af.absorb == 2) {
+        if (af.disp) rbt -= gp(l2)[off + idx];
+        dbar = gp(l1)[off + idx];
+      }
+      if (dir) {
+        float v;
+        if (csr_lookup(src, r, n, t, v)) {
+          gc += -2.f * src.fac[i] * v * rbt / sqrtf(c2);
+          rbt = 0.f;
+        }
+      }
+      rs += src.fac[i] * rbt;
+      if (af.nonlin) sb += -2.f * I.dt * s * gp(af.Dt)[(int64_t)i * nf + off + idx] * rbt;
+      gw(wo[i])[off + idx] = s * I.dt * r0 * nl * rbt - r0 * dbar;
+      rn[i] = s * s * rbt;
+    }
+    for (int i = 0; i < ND; ++i) gw(af.rb[i])[off + idx] = rn[i] + sb;
+    gw(af.cb)[off + idx] += gc;
+    if (srcon) gw(af.R)[off + idx] = rs;
+  }
+}
+
+// momentum (k_adj_mom) of axis ax: gin = Gop_ax w_ax on the grid, v out
+__global__ __launch_bounds__(kGenThreads) void k_gadj_mom(Geom g, AdjFields af, const Inst* __restrict__ inst, int t, int ax,
+                                                          const float* __restrict__ gin, float* __restrict__ vout) {
+  const int b = blockIdx.y;
+  const Inst& I = inst[b];
+  const int64_t off = (int64_t)b * g.ntot;
+  const Src& us = I.srcu[ax];
+  const bool uon = us.mode != SRC_NONE && t < us.len;
+  const bool udir = uon && us.mode == SRC_DIRICHLET;
+  float* const Ub = uon && !udir && af.U[ax] ? af.U[ax] + off : nullptr;
+  KW_GEN_ELEMENT_LOOP {
+    const int r = (int)(idx / g.nz), n = (int)(idx % g.nz);
+    const float s = gpml(I.pmlsg, g, ax, r, n);
+    float u = gp(af.ub[ax])[off + idx] + gp(gin)[off + idx];
+    if (udir) {
+      float v;
+      if (csr_lookup(us, r, n, t, v)) u = 0.f;
+    }
+    if (Ub) gw(Ub)[idx] = u;
+    gw(vout)[off + idx] = s * ld(I.dtr0[ax], idx, I.dtr0_s) * u;
+    gw(af.ub[ax])[off + idx] = s * s * u;
+  }
+}
+
+// pbar = sum_i Dop_i v_i (k_adj_sum), the terms on the grid
+__global__ __launch_bounds__(kGenThreads) void k_gadj_sum(Geom g, AdjFields af, const float* __restrict__ d0,
+                                                          const float* __restrict__ d1, const float* __restrict__ d2) {
+  const int b = blockIdx.y;
+  const int64_t off = (int64_t)b * g.ntot;
+  KW_GEN_ELEMENT_LOOP {
+    float a = gp(d0)[off + idx];
+    if (g.nd > 1) a += gp(d1)[off + idx];
+    if (g.nd > 2) a += gp(d2)[off + idx];
+    gw(af.pb)[off + idx] = a;
   }
 }
 
@@ -604,6 +731,53 @@ hipError_t launch_src_build_gen(const Geom& g, int nb, const Fields& fl, const I
   hipLaunchKernelGGL(k_gsrc_scatter, gen_elem_grid(g, nb), dim3(kGenThreads), 0, st, g, fl, inst, which, fl.tmp[3]);
   KW_GEN_CHECK(hipGetLastError());
   return gzfwd(g, nb, fl.tmp[3], g.ntot, fl.spec[buf], g.nspec, tw, fl.tdev, inst, which, st);
+}
+
+hipError_t launch_adj_abs1_gen(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw,
+                               hipStream_t st) {
+  if (!fl.tmp[0] || !fl.tmp[1]) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gadj_abs1, gen_elem_grid(g, nb), dim3(kGenThreads), 0, st, g, af, inst, fl.tmp[0], fl.tmp[1]);
+  KW_GEN_CHECK(hipGetLastError());
+  KW_GEN_CHECK(gzfwd(g, nb, fl.tmp[0], g.ntot, fl.spec[0], g.nspec, tw, nullptr, nullptr, GF_ALL, st));
+  if (af.disp) KW_GEN_CHECK(gzfwd(g, nb, fl.tmp[1], g.ntot, fl.spec[1], g.nspec, tw, nullptr, nullptr, GF_ALL, st));
+  return hipSuccess;
+}
+
+hipError_t launch_adj_eos_gen(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, int t,
+                              hipStream_t st) {
+  if (!fl.tmp[0] || !fl.tmp[2]) return hipErrorInvalidValue;
+  if (af.absorb == 2) {  // L1[tau w], L2[eta w] from spectra 0 and 1 (the fractional column passes' outputs)
+    if (!af.gl[0] || !af.gl[1]) return hipErrorInvalidValue;
+    KW_GEN_CHECK(gzinv(g, nb, fl.spec[0], g.nspec, af.gl[0], g.ntot, tw, 1.f, nullptr, inst, GF_ALL, st));
+    if (af.disp) KW_GEN_CHECK(gzinv(g, nb, fl.spec[1], g.nspec, af.gl[1], g.ntot, tw, 1.f, nullptr, inst, GF_ALL, st));
+  }
+  hipLaunchKernelGGL(k_gadj_eos, gen_elem_grid(g, nb), dim3(kGenThreads), 0, st, g, af, inst, t, af.gl[0], af.gl[1], fl.tmp[0],
+                     fl.tmp[1], fl.tmp[2]);
+  KW_GEN_CHECK(hipGetLastError());
+  for (int i = 0; i < g.nd; ++i)
+    KW_GEN_CHECK(gzfwd(g, nb, fl.tmp[i], g.ntot, fl.spec[i], g.nspec, tw, nullptr, nullptr, GF_ALL, st));
+  return hipSuccess;
+}
+
+hipError_t launch_adj_mom_gen(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, int t,
+                              hipStream_t st) {
+  if (!fl.tmp[0] || !fl.tmp[1]) return hipErrorInvalidValue;
+  for (int ax = 0; ax < g.nd; ++ax) {
+    KW_GEN_CHECK(gzinv(g, nb, fl.spec[ax], g.nspec, fl.tmp[0], g.ntot, tw, 1.f, nullptr, inst, GF_ALL, st));
+    hipLaunchKernelGGL(k_gadj_mom, gen_elem_grid(g, nb), dim3(kGenThreads), 0, st, g, af, inst, t, ax, fl.tmp[0], fl.tmp[1]);
+    KW_GEN_CHECK(hipGetLastError());
+    KW_GEN_CHECK(gzfwd(g, nb, fl.tmp[1], g.ntot, fl.spec[ax], g.nspec, tw, nullptr, nullptr, GF_ALL, st));
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_adj_sum_gen(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw,
+                              hipStream_t st) {
+  if (!fl.tmp[0] || !fl.tmp[2]) return hipErrorInvalidValue;
+  for (int i = 0; i < g.nd; ++i)
+    KW_GEN_CHECK(gzinv(g, nb, fl.spec[i], g.nspec, fl.tmp[i], g.ntot, tw, 1.f, nullptr, inst, GF_ALL, st));
+  hipLaunchKernelGGL(k_gadj_sum, gen_elem_grid(g, nb), dim3(kGenThreads), 0, st, g, af, fl.tmp[0], fl.tmp[1], fl.tmp[2]);
+  return hipGetLastError();
 }
 
 }  // namespace kw

@@ -22,6 +22,10 @@ hipError_t launch_eos_gen(const Geom& g, int nb, const Fields& fl, const Inst* i
 hipError_t launch_init_u_gen(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st);
 hipError_t launch_src_build_gen(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, int which, hipStream_t st);
 hipError_t launch_col_gen(const Geom& g, int nb, const Fields& fl, const Ops& ops, const Inst* inst, const ColArgs& a, int nline, hipStream_t st);
+hipError_t launch_adj_eos_gen(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, int t, hipStream_t st);
+hipError_t launch_adj_mom_gen(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, int t, hipStream_t st);
+hipError_t launch_adj_abs1_gen(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st);
+hipError_t launch_adj_sum_gen(const Geom& g, int nb, const Fields& fl, const AdjFields& af, const Inst* inst, const float2* tw, hipStream_t st);
 hipError_t launch_zfwd(const Geom& g, int nb, const float* f, int64_t fs, float2* spec, int64_t ss, const float2* tw, hipStream_t st);
 hipError_t launch_zinv(const Geom& g, int nb, const float2* spec, int64_t ss, float* f, int64_t fs, const float2* tw, float scale, hipStream_t st);
 hipError_t launch_momentum(const Geom& g, int nb, const Fields& fl, const Inst* inst, const float2* tw, hipStream_t st);

@@ -136,6 +136,7 @@ struct AdjFields {
   float* cb;         // accumulated dJ/dc0
   float* R;          // sum_i rb_i * fac_i on source-active steps
   float* U[3];       // ub_i after the step's mass-update adjoint (dJ/du_i^t), axes with an additive velocity source
+  float* gl[2];      // generic-length family, power law: L1[tau w] and L2[eta w] back on the grid
   const float* pt;   // forward p of the step being reversed
   const float* adj;  // dJ/d(recorded p), [nb][nrec][nsens]
   int nrec, nsens;

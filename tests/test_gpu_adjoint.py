@@ -8,7 +8,8 @@ adjoint source is dJ/dy = y - d.  Checks:
   in 2-D with all three pressure-source modes and in 3-D, with an initial pressure p0
   beside the line source and with p0 alone, in power-law (with and without
   dispersion) and Stokes media, with B/A nonlinearity (alone, and with p0 and
-  power-law absorption), and with velocity sources in each mode;
+  power-law absorption), with velocity sources in each mode, and on grids of the
+  generic-length family;
 * central finite differences of the GPU solver at 20 seeded cells (seed 7) of the
   128^2 surrogate problem, 600 steps, in the region the wave reaches and returns
   from.  h = 5 m/s: the FP32 solver's rounding noise in J (~1e4 of J ~ 1e13 over
@@ -121,6 +122,27 @@ def test_gpu_adjoint_velocity_source_matches_fp64_adjoint(shape, umode, p0, alph
     assert _rel(y, y_ref) < 1e-4
     err = _rel(g_gpu, g_ref)
     print(f"{shape} u:{umode} p0={p0} {alpha} B/A={nonlin}: gradient rel-L2 vs FP64 adjoint {err:.2e}")
+    assert err < 1e-3, err
+
+
+@pytest.mark.parametrize("shape,mode,p0,alpha,nonlin,umode", [
+    ((30, 44), "additive", False, None, False, None), ((33, 40), "dirichlet", True, "powerlaw", False, None),
+    ((30, 44), "additive", True, "powerlaw-mild", True, "additive"), ((18, 20, 22), "additive", False, "stokes", True, "dirichlet")])
+def test_gpu_adjoint_generic_lengths_match_fp64_adjoint(shape, mode, p0, alpha, nonlin, umode):
+    """Grids that run on the generic-length family: axis lengths with a prime factor above 7
+    (30 = 2 3 5 across the columns, 44 = 4 11 and 22 along the contiguous axis) or an odd
+    row count (33 rows), so the reverse step's contiguous-axis work is unfused and the
+    column passes use the stash spectrum."""
+    case = adjoint_case(shape=shape, nt=90 if len(shape) == 2 else 60, mode=mode, p0=p0, alpha=ABSORBING.get(alpha), u_source=umode,
+                        **(NONLINEAR if nonlin else {}))
+    y = _gpu_traces(case)
+    d = _misfit_data(y)
+    g_gpu, y2 = _gpu_grad(case, y - d, checkpoint=16)
+    assert np.array_equal(y, y2)
+    g_ref, y_ref = _oracle_grad(case, y - d)
+    assert _rel(y, y_ref) < 1e-4
+    err = _rel(g_gpu, g_ref)
+    print(f"{shape} {mode} p0={p0} {alpha} B/A={nonlin} u:{umode}: gradient rel-L2 vs FP64 adjoint {err:.2e}")
     assert err < 1e-3, err
 
 
