@@ -6,7 +6,7 @@ media, media with Stokes or power-law absorption and dispersion (:471-516; tau
 and eta depend on c0) and B/A nonlinearity (:542-550), with or without absorption,
 with additive or dirichlet pressure and velocity sources and/or an initial pressure p0 (whose t = 0 state,
 :729-733, depends on c0 through the initial densities p0 / (D c0^2)) and binary or
-Cartesian sensors: ``sound_speed_gradient`` returns dJ/dc0 for a given dJ/d(traces), and
+Cartesian sensors, in 1-D, 2-D and 3-D: ``sound_speed_gradient`` returns dJ/dc0 for a given dJ/d(traces), and
 ``sensor_traces`` wraps forward + adjoint as a ``torch.autograd.Function`` so a
 misfit written in torch can be minimised over the sound-speed map.
 
@@ -23,25 +23,34 @@ import copy
 import numpy as np
 
 from . import _abi
+from .solvers import embed1d
 from .solvers.plan import DevicePlan
 from .solvers.setup import build_instance
 
 
 def _plan(kgrid, medium, source, sensor, c, pml_size, pml_alpha, **kw):
+    """(device plan, set-up, 1-D?).  A 1-D problem runs as the 2 x N grid of solvers/embed1d.py,
+    c0 replicated into both rows."""
     if kw.get("source_scaled"):  # pre-scaled signals carry no c0 dependence the adjoint could use
         raise ValueError("adjoint: source_scaled signals are not differentiated")
     med = copy.copy(medium)
     if c is not None:
         med.sound_speed = np.asarray(c, dtype=np.float64)
+    one_d = embed1d.is_1d(kgrid)
+    if one_d:
+        kgrid, med, source, sensor, pml_size, pml_alpha = embed1d.embed(kgrid, med, source, sensor, pml_size, pml_alpha)
+        kw = dict(kw, smooth_p0=False)  # the reference does not smooth p0 in 1-D (kspace_solver.py:682)
     st = build_instance(kgrid, med, copy.deepcopy(source), sensor, pml_size=pml_size, pml_alpha=pml_alpha, **kw)
+    if one_d:
+        st = embed1d.adjust_setup(st)
     if "p" not in st.record:
         raise ValueError("the sensor must record 'p'")
-    return DevicePlan([st]), st
+    return DevicePlan([st]), st, one_d
 
 
 def forward_traces(kgrid, medium, source, sensor, *, c=None, pml_size=None, pml_alpha=None, **kw):
     """Recorded p of one simulation, (n_sensor, n_recorded_steps) float32."""
-    plan, st = _plan(kgrid, medium, source, sensor, c, pml_size, pml_alpha, **kw)
+    plan, st, _ = _plan(kgrid, medium, source, sensor, c, pml_size, pml_alpha, **kw)
     try:
         plan.run(st.Nt - plan.step_index(0))
         return plan.recorded(0, _abi.F_REC_P).copy()
@@ -52,9 +61,11 @@ def forward_traces(kgrid, medium, source, sensor, *, c=None, pml_size=None, pml_
 def sound_speed_gradient(kgrid, medium, source, sensor, adj_src, *, c=None, pml_size=None, pml_alpha=None,
                          checkpoint=50, **kw):
     """(dJ/dc0 as a grid, recorded p) for dJ/d(recorded p) = ``adj_src`` (n_sensor, n_rec)."""
-    plan, st = _plan(kgrid, medium, source, sensor, c, pml_size, pml_alpha, **kw)
+    plan, st, one_d = _plan(kgrid, medium, source, sensor, c, pml_size, pml_alpha, **kw)
     try:
         grad = plan.adjoint_c([adj_src], checkpoint=checkpoint)[0]
+        if one_d:  # c0 of a 1-D point sets both rows of the 2 x N grid
+            grad = grad.sum(axis=0)
         return grad, plan.recorded(0, _abi.F_REC_P).copy()
     finally:
         plan.close()

@@ -8,8 +8,8 @@ adjoint source is dJ/dy = y - d.  Checks:
   in 2-D with all three pressure-source modes and in 3-D, with an initial pressure p0
   beside the line source and with p0 alone, in power-law (with and without
   dispersion) and Stokes media, with B/A nonlinearity (alone, and with p0 and
-  power-law absorption), with velocity sources in each mode, and on grids of the
-  generic-length family;
+  power-law absorption), with velocity sources in each mode, on grids of the
+  generic-length family, and in 1-D;
 * central finite differences of the GPU solver at 20 seeded cells (seed 7) of the
   128^2 surrogate problem, 600 steps, in the region the wave reaches and returns
   from.  h = 5 m/s: the FP32 solver's rounding noise in J (~1e4 of J ~ 1e13 over
@@ -143,6 +143,29 @@ def test_gpu_adjoint_generic_lengths_match_fp64_adjoint(shape, mode, p0, alpha, 
     assert _rel(y, y_ref) < 1e-4
     err = _rel(g_gpu, g_ref)
     print(f"{shape} {mode} p0={p0} {alpha} B/A={nonlin} u:{umode}: gradient rel-L2 vs FP64 adjoint {err:.2e}")
+    assert err < 1e-3, err
+
+
+@pytest.mark.parametrize("mode,amp", [("additive", None), ("dirichlet", None), ("additive", 3e7)])
+def test_gpu_adjoint_1d_matches_fp64_adjoint(mode, amp):
+    """1-D (32 points, run as the 2 x N embedding, both rows' gradients summed): a point
+    pressure source beside p0, power law and an additive velocity source, linear or with
+    B/A = 6 at a source amplitude amp.  The sensor sits 15 points from the source, as in
+    the 2-D cases: on 64 points it would lie beyond the ~27 points a pulse covers in 90
+    steps and record only the sources' spectral tails, whose relative error is FP32
+    rounding noise (1.8e-4 in the traces there)."""
+    nl = dict(bona=6.0, amplitude=amp) if amp else {}
+    case = adjoint_case(shape=(32,), mode=mode, p0=True, alpha=ABSORBING["powerlaw"], u_source="additive", **nl)
+    y = _gpu_traces(case)
+    d = _misfit_data(y)
+    g_gpu, y2 = _gpu_grad(case, y - d, checkpoint=16)
+    assert np.array_equal(y, y2)
+    assert g_gpu.shape == (32,)
+    g_ref, y_ref = _oracle_grad(case, y - d)
+    fwd = _rel(y, y_ref)
+    err = _rel(g_gpu, g_ref)
+    print(f"1-D {mode} B/A amplitude {amp}: traces rel-L2 {fwd:.2e}, gradient rel-L2 vs FP64 adjoint {err:.2e}")
+    assert fwd < 1e-4
     assert err < 1e-3, err
 
 

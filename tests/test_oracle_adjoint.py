@@ -152,6 +152,28 @@ def test_oracle_adjoint_planted_sign_error_is_caught():
     assert abs(grad.ravel()[cell] - fd) > 1e-3 * abs(fd)
 
 
+@pytest.mark.parametrize("mode", ["additive", "dirichlet"])
+def test_oracle_adjoint_1d_matches_finite_differences(mode):
+    """1-D (64 points): a point pressure source beside p0, power law, B/A and an additive
+    velocity source, the same finite-difference bar as above."""
+    from oracle.kspace_adjoint import sound_speed_gradient
+
+    case = adjoint_case(shape=(64,), mode=mode, p0=True, alpha=ABSORBING["powerlaw"], u_source="additive", **NONLINEAR)
+    c = np.asarray(case[1].sound_speed, dtype=np.float64)
+    rec0 = _oracle(case, c).run()["p"]
+    d = 0.7 * rec0
+    grad, _ = sound_speed_gradient(_oracle(case, c), rec0 - d)
+    J = lambda cc: 0.5 * float(np.sum((_oracle(case, cc).run()["p"] - d) ** 2))  # noqa: E731
+    h = 1e-2
+    for cell in _cells(case, c):
+        cp, cm = c.copy(), c.copy()
+        cp.ravel()[cell] += h
+        cm.ravel()[cell] -= h
+        fd = (J(cp) - J(cm)) / (2 * h)
+        g = grad.ravel()[cell]
+        assert abs(g - fd) <= 1e-6 * max(abs(fd), 1e-12 * np.abs(grad).max()) + 1e-9 * np.abs(grad).max(), (cell, g, fd)
+
+
 @pytest.mark.parametrize("slip", ["speed term dropped", "points not cleared"])
 def test_oracle_adjoint_planted_dirichlet_slips_are_caught(slip):
     """A dirichlet pressure source overwrites every rho_i at its points with sig / (D c^2):
