@@ -860,6 +860,12 @@ def benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=36, max_err=0.02, m
     cal = sur.calibrate(xca, yca, policy="predicted", log=log)
     _, routed = sur.route(xte)
     after = np.where(routed, 0.0, err_tta)  # routed maps carry the solver's own traces
+    # the same routing without the mirror image: one forward per map, the head without the
+    # disagreement feature
+    sur_n = Surrogate(model, max_err=max_err, tta=False)
+    cal_n = sur_n.calibrate(xca, yca, policy="predicted", log=log)
+    _, routed_n = sur_n.route(xte)
+    after_n = np.where(routed_n, 0.0, err)
     # throughput, inputs resident on the device: surrogate forward (+ mirror, error head) vs the
     # batched solver's time loop; with the fallback, the routed fraction also pays the solver
     t_sur = time_surrogate(model, xte)
@@ -867,6 +873,8 @@ def benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=36, max_err=0.02, m
     t_sol = time_solver(xte.astype(np.float64))
     t_eff = t_sur_r + routed.mean() * t_sol
     t_eff_b = t_sur_r + routed_b.mean() * t_sol
+    t_sur_n = time_surrogate(model, xte, head=sur_n.head, tta=False)
+    t_eff_n = t_sur_n + routed_n.mean() * t_sol
     # host to host: c maps in host memory -> traces in host memory (the solver incl. its host set-up)
     t0 = time.perf_counter()
     predict(model, xte)
@@ -888,6 +896,10 @@ def benchmark(n_train=20000, n_test=1000, n_cal=3000, epochs=36, max_err=0.02, m
            "budget_policy": {"calibration": cal_b, "routed_frac": float(routed_b.mean()),
                              "after_routing_frac_above_2pct": float((after_b > max_err).mean()),
                              "speedup_with_fallback": None},
+           "no_mirror_policy": {"calibration": cal_n, "routed_frac": float(routed_n.mean()),
+                                "after_routing_rel_l2_mean": float(after_n.mean()),
+                                "after_routing_frac_above_2pct": float((after_n > max_err).mean()),
+                                "routed_surrogate_ms_per_sample": 1e3 * t_sur_n, "speedup_with_fallback": t_sol / t_eff_n},
            "after_routing_rel_l2_mean": float(after.mean()), "after_routing_rel_l2_p95": float(np.percentile(after, 95)),
            "after_routing_frac_above_2pct": float((after > max_err).mean()),
            "after_routing_output_rel_l2_max": float(err_out.max()),
