@@ -331,3 +331,13 @@ def test_oracle_adjoint_planted_nonlinear_slips_are_caught():
         fd = (J(cp) - J(cm)) / (2 * h)
         assert abs(good.ravel()[cell] - fd) <= 1e-6 * abs(fd), (name, good.ravel()[cell], fd)
         assert abs(bad.ravel()[cell] - fd) > 1e-3 * abs(fd), (name, bad.ravel()[cell], fd)
+
+
+def test_adjoint_rejects_prescaled_source_signals():
+    """Signals passed already scaled (the binaries' file route) carry no c0 dependence the
+    adjoint could differentiate: kwave_amd.adjoint refuses them before any device work."""
+    from kwave_amd.adjoint import sound_speed_gradient
+
+    kg, med, src, sen, pml, alpha = adjoint_case()
+    with pytest.raises(ValueError, match="source_scaled"):
+        sound_speed_gradient(kg, med, src, sen, np.zeros((1, 1)), pml_size=pml, pml_alpha=alpha, source_scaled=True)
