@@ -169,6 +169,43 @@ def test_gpu_adjoint_1d_matches_fp64_adjoint(mode, amp):
     assert err < 1e-3, err
 
 
+def test_gpu_adjoint_cartesian_sensor_is_the_transpose_of_its_interpolation():
+    """Off-grid Cartesian sensor points record W p, W the linear-interpolation weights of
+    their corner nodes: the traces equal W times the traces of a binary mask over those
+    corners, and the gradient for an adjoint source r equals the binary-mask gradient for
+    W^T r (the scatter of the reverse sweep is the gather's transpose; its atomic adds of
+    shared corners change the rounding only)."""
+    from kwave_amd.ksensor import kSensor
+    from kwave_amd.solvers.setup import linear_interp_table
+
+    kg, med, src, sen, pml, alpha = adjoint_case()
+    shape, dx = (32, 40), 1e-4
+    rng = np.random.default_rng(11)
+    pts = np.stack([(rng.uniform(20.0, 24.0, 12) - shape[0] // 2) * dx, (rng.uniform(8.0, 30.0, 12) - shape[1] // 2) * dx])
+    idx, w = linear_interp_table(pts, shape, (dx, dx))
+    corners = np.unique(idx.ravel())  # a binary mask records in ascending flat index
+    W = np.zeros((pts.shape[1], corners.size))
+    for p in range(pts.shape[1]):
+        for c in range(idx.shape[1]):
+            W[p, np.searchsorted(corners, idx[p, c])] += w[p, c]
+    sen_c = kSensor(pts)
+    sen_c.record = ["p"]
+    bm = np.zeros(int(np.prod(shape)), dtype=bool)
+    bm[corners] = True
+    sen_b = kSensor(bm.reshape(shape))
+    sen_b.record = ["p"]
+    case_c = (kg, med, src, sen_c, pml, alpha)
+    case_b = (kg, med, src, sen_b, pml, alpha)
+    yc, yb = _gpu_traces(case_c), _gpu_traces(case_b)
+    assert _rel(yc, W @ yb) < 1e-6
+    r = yc - _misfit_data(yc)
+    gc, _ = _gpu_grad(case_c, r, checkpoint=16)
+    gb, _ = _gpu_grad(case_b, (W.T @ r).astype(np.float32), checkpoint=16)
+    err = _rel(gc, gb)
+    print(f"Cartesian vs binary-corner gradient: rel-L2 {err:.2e}")
+    assert err < 1e-5, err
+
+
 def test_gpu_adjoint_matches_finite_differences_surrogate_problem():
     from kwave_amd import workloads
 
