@@ -426,12 +426,18 @@ __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MASS) void k_mass(Geo
   // density update (dt rho0 when linear; rho_old = sum of the split densities
   // of the previous step, kept in rhot), and the running sums of div u and of
   // the new densities
-  float2 qv[C::E], divu[C::E], rt[C::E];
+  // rho0 stays in registers for the outputs at the end where that costs no occupancy (3-D,
+  // power-of-two lengths: two waves per SIMD either way).  A re-read there misses L2 at
+  // large grids, about 4 bytes per voxel of extra HBM traffic; the 2-D kernels would drop
+  // from three waves per SIMD to two, the mixed-radix ones from two to one.
+  constexpr bool R0R = ND == 3 && C::E <= 16 && (N & (N - 1)) == 0;
+  float2 qv[C::E], divu[C::E], rt[C::E], r0v[R0R ? C::E : 1];
   float2 ch[SREG ? C::E : 1];
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
     const int64_t i0 = iA + m * C::TPL;
     float qa = ld(I.rho0, i0, I.rho0_s), qb = ld(I.rho0, i0 + N, I.rho0_s);
+    if constexpr (R0R) r0v[m] = make_float2(qa, qb);
     if (flags & FLAG_NONLINEAR) {
       qa += 2.f * gw(fl.rhot)[off + i0];
       qb += 2.f * gw(fl.rhot)[off + i0 + N];
@@ -532,8 +538,9 @@ __global__ __launch_bounds__(ZCfg<N>::NT) KW_WPE(N, KW_WPE_MASS) void k_mass(Geo
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int64_t idx = iA + h * N + m * C::TPL;
-      // rho0 again: this block read the same lines at its start, so they come from L2
-      const float r0 = ld(I.rho0, idx, I.rho0_s);
+      float r0;
+      if constexpr (R0R) r0 = h ? r0v[m].y : r0v[m].x;
+      else r0 = ld(I.rho0, idx, I.rho0_s);  // the lines this block read at its start, from L2 if still there
       const float du = h ? divu[m].y : divu[m].x;
       const float rtv = h ? rt[m].y : rt[m].x;
       if (powerlaw) {
